@@ -1,0 +1,193 @@
+"""IS-SPA MD throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--cpu-threads T] [--no-cpu]
+
+A step is one full MD step of the benchmark workload -- bonded terms, IS-SPA
+MC points + field + force passes, all-pairs nonbond, leap-frog + Andersen --
+replayed from a captured hipGraph with every input resident in HBM.  Prints
+ONE JSON line on stdout (diagnostics go to stderr).
+
+Roofline: the dominant kernel is the IS-SPA force pass.  Its algorithmic
+FLOPs per launch are 70 per (atom, point) pair (the counting convention of
+the workload definition: add/mul 1, FMA 2, rsqrt 1, compares and selects 0,
+per-point and per-atom constants hoisted), i.e. 70 N P; the average launch
+time comes from HIP events around back-to-back launches on the stream the
+kernel runs on (isspa_model_profile).  Peak: 157.3 TFLOP/s FP32 vector.
+
+CPU baseline: the reference's own src/isspa_force.omp.c (built in place into
+oracle/_ref by oracle/Makefile), cpu_ispa_force on the same 19,998-atom
+geometry with nMC=20 and the workload's per-type parabola and LJ-solvent
+parameters, timed on this host's cores.  It computes a cheaper LJ-only,
+single-pass model; it is a reported baseline, not the target.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "isspa-force-cuda_amd"))
+
+METRIC = "IS-SPA MD steps/s at 20k atoms nMC=20; batched 64-window ns/day vs plain"
+FP32_PEAK = 157.3e12
+FLOP_FORCE_PAIR = 70      # force pass, per (atom, point), outside rmax
+FLOP_FIELD_PAIR = 24      # field pass, per (point, atom), outside rmax
+FLOP_NB_PAIR = 36         # nonbond, per unordered pair
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(system, threads, budget_s=30.0):
+    """The reference OpenMP cpu_ispa_force on the workload geometry.  Returns
+    (seconds per call, kind, trials) or None if neither library is built."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    ref = po.ref_lib()
+    kind = "reference"
+    fn = ref.cpu_ispa_force if ref is not None else None
+    if fn is None:
+        fn, kind = po.lib().orc_cpu_ispa_force, "port"
+    par = system.type_params
+    T = len(par)
+    w = np.array([par[t]["w"] for t in range(T)], np.float32)
+    x0 = np.array([par[t]["x0"] for t in range(T)], np.float32)
+    g0 = np.array([par[t]["g0"] for t in range(T)], np.float32)
+    al = np.array([par[t]["alpha"] for t in range(T)], np.float32)
+    gr2 = np.array([[par[t]["r_lo"] ** 2, par[t]["r_hi"] ** 2] for t in range(T)], np.float32).ravel()
+    A = np.array([par[t]["lj_A"] for t in range(T)], np.float32)
+    B = np.array([par[t]["lj_B"] for t in range(T)], np.float32)
+    xyz = np.ascontiguousarray(system.xyzq[:, :3]).ravel()
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        po.cpu_ispa_force(fn, xyz, w, x0, g0, gr2, al, A, B, system.ityp, system.n_mc, threads,
+                          seed=1)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu baseline trial {len(times)}: {times[-1]:.2f} s ({threads} threads, {kind})")
+        if len(times) >= 3 or time.perf_counter() - t_start + times[-1] > budget_s:
+            break
+    times.sort()
+    return times[len(times) // 2], kind, len(times)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-reps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the per-GPU shard design is documented, no multi-GPU path runs here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from isspa_amd import synth
+    from isspa_amd.engine import GpuSystem
+
+    t0 = time.perf_counter()
+    s = synth.config(args.config)
+    log(f"workload config[{args.config}]: N={s.n_atoms} nMC={s.n_mc} P={s.n_points} "
+        f"box={s.lbox} built in {time.perf_counter() - t0:.1f}s")
+    g = GpuSystem(s, "cuda:0")
+    splits, chunks = g.tiling()
+    g.set_step(0)
+    g.capture(1)
+    g.replay(args.warmup)
+    torch.cuda.synchronize()
+    side = g._side()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record(side)
+    g.replay(args.steps)
+    ev1.record(side)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ev_ms = ev0.elapsed_time(ev1)
+    x = g.x.cpu().numpy()
+    if not np.all(np.isfinite(x)):
+        log("non-finite positions after the timed steps")
+        sys.exit(1)
+    steps_per_s = args.steps / wall
+    ms_step = 1e3 * wall / args.steps
+    log(f"timed {args.steps} steps: wall {wall:.3f}s  events {ev_ms / 1e3:.3f}s  "
+        f"{steps_per_s:.2f} steps/s  {ms_step:.3f} ms/step")
+
+    prof = g.profile(args.profile_reps)
+    log("stage ms: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()))
+    N, P = s.n_atoms, s.n_points
+    force_flop = FLOP_FORCE_PAIR * N * P
+    achieved = force_flop / (prof["force"] * 1e-3)
+    field_tflops = FLOP_FIELD_PAIR * N * P / (prof["field"] * 1e-3) / 1e12
+    step_flop = (FLOP_FIELD_PAIR + FLOP_FORCE_PAIR) * N * P + FLOP_NB_PAIR * N * (N - 1) / 2
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"config[{args.config}]: {N // 22} seeded copies of the 22-atom chain "
+                        f"({N} atoms), nMC={s.n_mc}, box {s.lbox:g} A; bonded + IS-SPA + "
+                        "all-pairs nonbond + leap-frog/Andersen, one hipGraph replay per step",
+            "n_atoms": N, "n_mc": s.n_mc, "mc_points": P, "dt_fs": 2.0,
+            "field_splits": splits, "force_chunks": chunks,
+        },
+        "ns_per_day": round(0.1728 * steps_per_s, 4),
+        "step_tflops": round(step_flop * steps_per_s / 1e12, 3),
+        "roofline": {
+            "kernel": "k_force (IS-SPA force pass)",
+            "bound": "valu",
+            "achieved": round(achieved / 1e12, 3),
+            "peak": FP32_PEAK / 1e12,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK, 4),
+            "traffic": None,
+            "flop_per_launch": force_flop,
+            "launch_ms": round(prof["force"], 4),
+        },
+        "stage_ms": {k: round(v, 4) for k, v in prof.items()},
+        "field_pass_tflops": round(field_tflops, 3),
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        res = cpu_baseline(s, threads)
+        if res is not None:
+            t_cpu, kind, trials = res
+            out["cpu_baseline"] = {
+                "value": round(1.0 / t_cpu, 5), "unit": "steps/s", "cores": threads,
+                "kind": kind,
+                "sample": f"cpu_ispa_force (src/isspa_force.omp.c) once per trial on the same "
+                          f"{N}-atom geometry, nMC={s.n_mc}, median of {trials} trial(s); "
+                          "LJ-only single-pass model",
+                "seconds_per_step": round(t_cpu, 3),
+            }
+            out["gpu_over_cpu_step_ratio"] = round(t_cpu / (ms_step * 1e-3), 1)
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,486 @@
+// IS-SPA solvent kernels for gfx950 (CDNA4): MC points, field pass, per-point
+// epilogue, force pass, chunk reduction.
+//
+// Semantics follow src/isspa_force_cuda.cu:73-327 of the reference; the
+// decomposition does not.  The reference launches one thread per (point,
+// atom) pair and folds results with wave32 shuffles and float atomics.  Here:
+//
+//  * field pass: one lane owns one MC point; the atom stream is wave-uniform
+//    (scalar loads), so each pair costs ~14 VALU ops + one v_rsq.  In-sphere
+//    pairs (<2% of wave iterations) take a skipped-when-empty branch.
+//  * epilogue: per point, in FP64: |E|, E-hat, Langevin moments by series or
+//    recurrence (no tanh cancellation, quirk Q22), and the hoisted constants
+//    of the force pass packed in a 64-byte record.
+//  * force pass: one lane owns one receiving atom; the point-record stream is
+//    wave-uniform.  Per-atom factors (-q_i vtot_i, -vtot_i) are applied once
+//    in the reduction.  The CDD term is taken for every pair in the main path
+//    and undone in the rare in-sphere branch, so the main path has no select.
+//  * no float atomics: lanes keep FP32 partials over short tiles, flush them
+//    into FP64, and write per-chunk partials that a reduction kernel sums in a
+//    fixed order (quirk Q10).
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace isspa {
+
+constexpr int kBlock = 256;
+constexpr int kFieldTile = 128;
+constexpr int kForceTile = 64;
+
+// x + R * (v 2^-24) with two roundings, never fused into one FMA, so the
+// oracle forms the same bits (v 2^-24 itself is exact)
+__device__ __forceinline__ float point_coord(float x, float R, int v)
+{
+#pragma clang fp contract(off)
+    const float u = (float)v * 5.9604644775390625e-08f;
+    const float du = R * u;
+    return x + du;
+}
+
+// ---------------------------------------------------------------------------
+// A.1 MC points (src/isspa_force_cuda.cu:73-107), Philox instead of curand.
+// .w of each point carries the owner's rmax^2 for the field pass.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_mc_points(
+    const float4 *__restrict__ xyzq, const int *__restrict__ itype, const float *__restrict__ rmax,
+    const float *__restrict__ rmax2, int n, int nmc, uint64_t seed, uint32_t sys,
+    uint32_t step_arg, const uint32_t *__restrict__ step_d, float4 *__restrict__ pts)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= n * nmc) return;
+    const uint32_t step = step_d ? *step_d : step_arg;
+    const int k = m / nmc;
+    const int t = itype[k];
+    int v0 = 0, v1 = 0, v2 = 0;
+    for (int a = 0; a < kMcMaxAttempts; ++a) {
+        const uint4 o = philox_draw(seed, (uint32_t)m, kStreamPoints, (uint32_t)a, step, sys);
+        const int c0 = 2 * (int)(o.x >> 8) - (1 << 24);
+        const int c1 = 2 * (int)(o.y >> 8) - (1 << 24);
+        const int c2 = 2 * (int)(o.z >> 8) - (1 << 24);
+        const uint64_t r2 = (uint64_t)((int64_t)c0 * c0 + (int64_t)c1 * c1 + (int64_t)c2 * c2);
+        if (r2 <= kMcAcceptMax) {
+            v0 = c0; v1 = c1; v2 = c2;
+            break;
+        }
+    }
+    const float R = rmax[t];
+    const float4 x = xyzq[k];
+    float4 p;
+    p.x = point_coord(x.x, R, v0);
+    p.y = point_coord(x.y, R, v1);
+    p.z = point_coord(x.z, R, v2);
+    p.w = rmax2[t];
+    pts[m] = p;
+}
+
+// ---------------------------------------------------------------------------
+// A.2 field pass (src/isspa_force_cuda.cu:110-206)
+// ---------------------------------------------------------------------------
+struct FieldRare {
+    float g;
+    int cnt;
+    double ex, ey, ez;
+};
+
+__device__ __forceinline__ void field_in_sphere(const TableView &tb, int tj, float d2, float inv,
+                                                float s, float rx, float ry, float rz, float e0,
+                                                FieldRare &st)
+{
+    const float d = d2 * inv;
+    const float x = (d - tb.g_r0) * tb.g_inv;
+    const int b = (int)x;  // truncation, as the reference's int()
+    float fac = 1.0f, et = 0.0f;
+    if (b < 0) {
+        fac = 0.0f;
+    } else if (b < tb.n_g - 1) {
+        const float ph = (d - (tb.g_r0 + (float)b * tb.g_dr)) * tb.g_inv;
+        const float *G = tb.g + tj * tb.ld_g;
+        fac = __builtin_fmaf(G[b + 1], ph, G[b] * (1.0f - ph));
+        const float phe = (d - (tb.e_r0 + (float)b * tb.e_dr)) * tb.e_inv;
+        const float *Et = tb.e + tj * tb.ld_e;
+        et = __builtin_fmaf(Et[b + 1], phe, Et[b] * (1.0f - phe));
+    }
+    st.g *= fac;
+    st.cnt += 1;
+    const float c = __builtin_fmaf(-e0, s, et * inv);
+    st.ex += (double)(c * rx);
+    st.ey += (double)(c * ry);
+    st.ez += (double)(c * rz);
+}
+
+template <bool PBC>
+__device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, float px, float py,
+                                           float pz, float R2, float &ax, float &ay, float &az,
+                                           FieldRare &st)
+{
+    const float4 *__restrict__ xq = a.xyzq;
+#pragma unroll 4
+    for (int j = j0; j < j1; ++j) {
+        const float4 q = xq[j];  // wave-uniform
+        float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
+        if (PBC) {
+            rx = min_image1(rx, a.L, a.h);
+            ry = min_image1(ry, a.L, a.h);
+            rz = min_image1(rz, a.L, a.h);
+        }
+        const float d2 = dist2_f32(rx, ry, rz);
+        const float inv = rsqrt_f32(d2);
+        const float s = q.w * (inv * inv * inv);
+        const bool in = d2 <= R2;
+        const float so = in ? 0.0f : s;
+        ax = __builtin_fmaf(so, rx, ax);
+        ay = __builtin_fmaf(so, ry, ay);
+        az = __builtin_fmaf(so, rz, az);
+        if (in) field_in_sphere(a.tab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0, st);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    const int split = blockIdx.y;
+    const int j0 = split * a.atoms_per_split;
+    const int j1 = min(a.n, j0 + a.atoms_per_split);
+    const bool valid = m < a.P;
+    const float4 p = valid ? a.pts[m] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    const bool pbc = *a.pbc_flag != 0;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    FieldRare st{1.0f, 0, 0.0, 0.0, 0.0};
+    for (int jt = j0; jt < j1; jt += kFieldTile) {
+        const int je = min(j1, jt + kFieldTile);
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (pbc)
+            field_tile<true>(a, jt, je, p.x, p.y, p.z, p.w, ax, ay, az, st);
+        else
+            field_tile<false>(a, jt, je, p.x, p.y, p.z, p.w, ax, ay, az, st);
+        sx += (double)ax;
+        sy += (double)ay;
+        sz += (double)az;
+    }
+    if (!valid) return;
+    const size_t P = (size_t)a.P;
+    a.part_g[split * P + m] = st.g;
+    a.part_cnt[split * P + m] = st.cnt;
+    double *s0 = a.part_s0 + (size_t)split * 3 * P;
+    double *ei = a.part_ein + (size_t)split * 3 * P;
+    s0[m] = sx;
+    s0[P + m] = sy;
+    s0[2 * P + m] = sz;
+    ei[m] = st.ex;
+    ei[P + m] = st.ey;
+    ei[2 * P + m] = st.ez;
+}
+
+// ---------------------------------------------------------------------------
+// per-point epilogue in FP64: Langevin moments <P1>,<P2>,<P3>
+// (= c1, 1.5 c2 - 0.5, 2.5 c3 - 1.5 c1 of src/isspa_force_cuda.cu:270-287)
+// ---------------------------------------------------------------------------
+// Below x = 5: <P_l> = i_l(x)/i_0(x) from the all-positive power series
+// i_l(x) = x^l sum_k (x^2/2)^k / (k! (2l+2k+1)!!), 30 terms; above 5 the
+// upward recurrence from coth, which then cancels by at most ~3x.
+__device__ __forceinline__ void langevin_moments(double x, double &m1, double &m2, double &m3)
+{
+    if (x < 5.0) {
+        const double h = 0.5 * x * x;
+        double s[4];
+        double first = 1.0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            if (l > 0) first /= (double)(2 * l + 1);
+            double t = first, sum = first;
+            for (int k = 0; k < 30; ++k) {
+                t *= h / ((double)(k + 1) * (double)(2 * l + 2 * k + 3));
+                sum += t;
+            }
+            s[l] = sum;
+        }
+        m1 = x * s[1] / s[0];
+        m2 = x * x * s[2] / s[0];
+        m3 = x * x * x * s[3] / s[0];
+        return;
+    }
+    const double coth = 1.0 / tanh(x);
+    m1 = coth - 1.0 / x;
+    m2 = 1.0 - 3.0 * m1 / x;
+    m3 = m1 - 5.0 * m2 / x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
+{
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= a.P) return;
+    const size_t P = (size_t)a.P;
+    double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
+    int cnt = 0;
+    for (int s = 0; s < a.splits; ++s) {
+        g *= (double)a.part_g[s * P + m];
+        cnt += a.part_cnt[s * P + m];
+        for (int c = 0; c < 3; ++c) {
+            s0[c] += a.part_s0[(s * 3 + c) * P + m];
+            ei[c] += a.part_ein[(s * 3 + c) * P + m];
+        }
+    }
+    double E0[3], E[3];
+    for (int c = 0; c < 3; ++c) {
+        E0[c] = -(double)a.e0 * s0[c];
+        E[c] = E0[c] + ei[c];
+    }
+    const double x = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+    double eh[3];
+    for (int c = 0; c < 3; ++c) eh[c] = x > 0.0 ? E[c] / x : 0.0;
+    double m1, m2, m3;
+    langevin_moments(x, m1, m2, m3);
+    // cnt >= 1 always: the owner atom lies inside its own sphere
+    const double inv_cnt = 1.0 / (double)(cnt > 0 ? cnt : 1);
+    const double w = g * inv_cnt;
+    const double cp = (double)a.p0 * inv_cnt;
+    const double A1 = (double)a.p0 * m1 * w;
+    const double A2 = (double)a.q0 * m2 * w;
+    const double A3 = (double)a.o0 * m3 * w;
+    const float4 p = a.pts[m];
+    float *r = a.rec + (size_t)m * kRecFloats;
+    float4 *r4 = reinterpret_cast<float4 *>(r);
+    r4[0] = make_float4(p.x, p.y, p.z, (float)eh[0]);
+    r4[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + cp * E0[0] / 3.0),
+                        (float)(A1 * eh[1] + cp * E0[1] / 3.0));
+    r4[2] = make_float4((float)(A1 * eh[2] + cp * E0[2] / 3.0), (float)(cp * E0[0]),
+                        (float)(cp * E0[1]), (float)(cp * E0[2]));
+    r4[3] = make_float4((float)A1, (float)A2, (float)A3, (float)w);
+    if (a.g_out) a.g_out[m] = g;
+    if (a.cnt_out) a.cnt_out[m] = cnt;
+    if (a.E_out)
+        for (int c = 0; c < 3; ++c) a.E_out[3 * m + c] = E[c];
+    if (a.E0_out)
+        for (int c = 0; c < 3; ++c) a.E0_out[3 * m + c] = E0[c];
+}
+
+// ---------------------------------------------------------------------------
+// A.3 force pass (src/isspa_force_cuda.cu:208-327)
+//
+// For receiver i and point m (r = p_m - x_i, u = Ehat.r/d, P2 = 7.5u^2-1.5,
+// P3 = (17.5u^2-7.5)u), the reference's three multipole terms plus the CDD
+// term, with -q_i vtot_i factored out, are
+//     S += alpha r - d^-3 V - d^-4 k Ehat
+//     alpha = d^-4 (3 A1 u + d^-1 (A2 P2 + d^-1 A3 P3)) + d^-5 (E0c.r)
+//     k     = 3 u A2 + d^-1 P2 A3
+// with V = A1 Ehat + cp E0/3 and E0c = cp E0.  Inside the receiver's sphere
+// the CDD part (d^-5 (E0c.r) r - d^-3 E0c/3) is removed again and the LJ
+// table term is added instead (with -vtot_i factored out).
+// ---------------------------------------------------------------------------
+struct ForceAcc {
+    double ex, ey, ez;  // electrostatic (pol + CDD) sum, or pol in split mode
+    double lx, ly, lz;  // LJ sum
+    double cx, cy, cz;  // CDD sum (split mode only)
+};
+
+template <bool PBC, bool SPLIT>
+__device__ __forceinline__ void force_tile(const ForceArgs &a, int m0, int m1, float xi, float yi,
+                                           float zi, float R2, const float *__restrict__ Ftab,
+                                           float &sx, float &sy, float &sz, ForceAcc &acc)
+{
+    const float4 *__restrict__ rec = a.rec4;
+#pragma unroll 2
+    for (int m = m0; m < m1; ++m) {
+        const float4 q0 = rec[4 * m + 0];  // wave-uniform
+        const float4 q1 = rec[4 * m + 1];
+        const float4 q2 = rec[4 * m + 2];
+        const float4 q3 = rec[4 * m + 3];
+        const float Ex = q0.w, Ey = q1.x, Ez = q1.y;
+        const float Vx = q1.z, Vy = q1.w, Vz = q2.x;
+        const float Cx = q2.y, Cy = q2.z, Cz = q2.w;
+        const float A1 = q3.x, A2 = q3.y, A3 = q3.z, w = q3.w;
+        float rx = q0.x - xi, ry = q0.y - yi, rz = q0.z - zi;
+        if (PBC) {
+            rx = min_image1(rx, a.L, a.h);
+            ry = min_image1(ry, a.L, a.h);
+            rz = min_image1(rz, a.L, a.h);
+        }
+        const float d2 = dist2_f32(rx, ry, rz);
+        const float rinv = rsqrt_f32(d2);
+        const float ue = __builtin_fmaf(Ez, rz, __builtin_fmaf(Ey, ry, Ex * rx));
+        const float ec = __builtin_fmaf(Cz, rz, __builtin_fmaf(Cy, ry, Cx * rx));
+        const float u = ue * rinv;
+        const float rinv2 = rinv * rinv;
+        const float rinv3 = rinv2 * rinv;
+        const float rinv4 = rinv2 * rinv2;
+        const float u2 = u * u;
+        const float P2 = __builtin_fmaf(7.5f, u2, -1.5f);
+        const float P3 = __builtin_fmaf(17.5f, u2, -7.5f) * u;
+        const float u3 = 3.0f * u;
+        float hh = __builtin_fmaf(A2, P2, (A3 * P3) * rinv);
+        hh = __builtin_fmaf(hh, rinv, A1 * u3);
+        const float k = __builtin_fmaf(P2 * A3, rinv, u3 * A2) * rinv4;
+        const bool in = d2 <= R2;
+        if (!SPLIT) {
+            const float alpha = __builtin_fmaf(ec, rinv, hh) * rinv4;
+            sx = __builtin_fmaf(alpha, rx, sx);
+            sy = __builtin_fmaf(alpha, ry, sy);
+            sz = __builtin_fmaf(alpha, rz, sz);
+            sx = __builtin_fmaf(-rinv3, Vx, sx);
+            sy = __builtin_fmaf(-rinv3, Vy, sy);
+            sz = __builtin_fmaf(-rinv3, Vz, sz);
+            sx = __builtin_fmaf(-k, Ex, sx);
+            sy = __builtin_fmaf(-k, Ey, sy);
+            sz = __builtin_fmaf(-k, Ez, sz);
+        } else {
+            const float ap = hh * rinv4;
+            const float bp = __builtin_fmaf(rinv3, A1, k);
+            sx = __builtin_fmaf(ap, rx, __builtin_fmaf(-bp, Ex, sx));
+            sy = __builtin_fmaf(ap, ry, __builtin_fmaf(-bp, Ey, sy));
+            sz = __builtin_fmaf(ap, rz, __builtin_fmaf(-bp, Ez, sz));
+            if (!in) {
+                const float ac = ec * rinv * rinv4;
+                const float bc = rinv3 * (1.0f / 3.0f);
+                acc.cx += (double)__builtin_fmaf(ac, rx, -bc * Cx);
+                acc.cy += (double)__builtin_fmaf(ac, ry, -bc * Cy);
+                acc.cz += (double)__builtin_fmaf(ac, rz, -bc * Cz);
+            }
+        }
+        if (in) {
+            const float d = d2 * rinv;
+            const int b = (int)((d - a.tab.f_r0) * a.tab.f_inv + 0.5f);
+            float fs = 0.0f;
+            if (b >= 0 && b < a.tab.n_f) {
+                const float ph = (d - (a.tab.f_r0 + (float)b * a.tab.f_dr)) * a.tab.f_inv;
+                fs = __builtin_fmaf(Ftab[b + 1], ph, Ftab[b] * (1.0f - ph)) * w;
+            }
+            const float fl = fs * rinv;
+            acc.lx += (double)(fl * rx);
+            acc.ly += (double)(fl * ry);
+            acc.lz += (double)(fl * rz);
+            if (!SPLIT) {
+                // remove the CDD part taken in the main path
+                const float ac = ec * rinv * rinv4;
+                const float bc = rinv3 * (1.0f / 3.0f);
+                acc.ex -= (double)__builtin_fmaf(ac, rx, -bc * Cx);
+                acc.ey -= (double)__builtin_fmaf(ac, ry, -bc * Cy);
+                acc.ez -= (double)__builtin_fmaf(ac, rz, -bc * Cz);
+            }
+        }
+    }
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int m0 = chunk * a.pts_per_chunk;
+    const int m1 = min(a.P, m0 + a.pts_per_chunk);
+    const bool valid = i < a.n;
+    const float4 xi = valid ? a.xyzq[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int ti = valid ? a.itype[i] : 0;
+    const float R2 = valid ? a.rmax2[ti] : -1.0f;
+    const float *Ftab = a.tab.f + ti * a.tab.ld_f;
+    const bool pbc = *a.pbc_flag != 0;
+    ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int mt = m0; mt < m1; mt += kForceTile) {
+        const int me = min(m1, mt + kForceTile);
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        if (pbc)
+            force_tile<true, SPLIT>(a, mt, me, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
+        else
+            force_tile<false, SPLIT>(a, mt, me, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
+        acc.ex += (double)sx;
+        acc.ey += (double)sy;
+        acc.ez += (double)sz;
+    }
+    if (!valid) return;
+    const size_t N = (size_t)a.n;
+    const int comps = SPLIT ? 9 : 6;
+    double *o = a.part + (size_t)chunk * comps * N;
+    o[0 * N + i] = acc.ex;
+    o[1 * N + i] = acc.ey;
+    o[2 * N + i] = acc.ez;
+    o[3 * N + i] = acc.lx;
+    o[4 * N + i] = acc.ly;
+    o[5 * N + i] = acc.lz;
+    if (SPLIT) {
+        o[6 * N + i] = acc.cx;
+        o[7 * N + i] = acc.cy;
+        o[8 * N + i] = acc.cz;
+    }
+}
+
+// F_i = -q_i vtot_i S_e - vtot_i S_lj, summed over chunks in chunk order.
+template <bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const size_t N = (size_t)a.n;
+    const int comps = SPLIT ? 9 : 6;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < a.chunks; ++c) {
+        const double *p = a.part + (size_t)c * comps * N;
+        for (int k = 0; k < comps; ++k) s[k] += p[k * N + i];
+    }
+    const float4 x = a.xyzq[i];
+    const double vt = (double)a.vtot[a.itype[i]];
+    const double qv = -(double)x.w * vt;
+    if (!SPLIT) {
+        float4 f = a.f[i];
+        f.x += (float)(qv * s[0] - vt * s[3]);
+        f.y += (float)(qv * s[1] - vt * s[4]);
+        f.z += (float)(qv * s[2] - vt * s[5]);
+        a.f[i] = f;
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            a.f_pol[3 * i + c] = qv * s[c];
+            a.f_lj[3 * i + c] = -vt * s[3 + c];
+            a.f_cdd[3 * i + c] = qv * s[6 + c];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+hipError_t launch_mc_points(const float4 *xyzq, const int *itype, const float *rmax,
+                            const float *rmax2, int n, int nmc, uint64_t seed, uint32_t sys,
+                            uint32_t step, const uint32_t *step_d, float4 *pts, hipStream_t s)
+{
+    const long P = (long)n * nmc;
+    if (P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P)), dim3(kBlock), 0, s, xyzq, itype, rmax,
+                       rmax2, n, nmc, seed, sys, step, step_d, pts);
+    return hipGetLastError();
+}
+
+hipError_t launch_field(const FieldArgs &a, int splits, hipStream_t s)
+{
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field, dim3(blocks_for(a.P), splits), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s)
+{
+    if (a.P == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field_epilogue, dim3(blocks_for(a.P)), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const ForceArgs &a, int chunks, bool split, hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    if (split)
+        hipLaunchKernelGGL(k_force<true>, dim3(blocks_for(a.n), chunks), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_force<false>, dim3(blocks_for(a.n), chunks), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    if (split)
+        hipLaunchKernelGGL(k_force_reduce<true>, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_force_reduce<false>, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace isspa

@@ -1,0 +1,607 @@
+// Nonbond, bonded, integrator, restraint and utility kernels for gfx950.
+//
+// Semantics follow the reference kernels cited at each function; the
+// decomposition is this engine's own: one lane per receiving atom with a
+// wave-uniform partner stream for the all-pairs term, per-term contribution
+// slots plus a per-atom gather for the bonded terms (no float atomics), and
+// single-workgroup FP64 tree reductions for the umbrella centres of mass.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace isspa {
+
+constexpr int kBlk = 256;
+static inline int nblocks(long n) { return (int)((n + kBlk - 1) / kBlk); }
+
+// ---------------------------------------------------------------------------
+// A.4 all-pairs nonbond (src/nonbond_force_cuda.cu:31-174):
+//   non-excluded pairs: (12A/d^12 - 6B/d^6)/d^2 + q_i q_j/d^3
+//   all pairs:          IS-SPA pair electrostatics fdir(d, R_j)
+// F_i += s r with r = min_image(x_i - x_j).
+// ---------------------------------------------------------------------------
+template <bool PBC, bool SPLIT>
+__device__ __forceinline__ void nonbond_tile(const NonbondArgs &a, int i, int j0, int j1,
+                                             float4 xi, int ti, const uint32_t *ex, bool near,
+                                             const float2 *__restrict__ ljrow, double *acc)
+{
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;           // total (or LJ in split mode)
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f;           // Coulomb (split)
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f;           // pair term (split)
+    for (int j = j0; j < j1; ++j) {
+        const float4 xj = a.xyzq[j];  // wave-uniform
+        const float4 aj = a.aux[j];
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        if (PBC) {
+            rx = min_image1(rx, a.L, a.h);
+            ry = min_image1(ry, a.L, a.h);
+            rz = min_image1(rz, a.L, a.h);
+        }
+        const float d2 = dist2_f32(rx, ry, rz);
+        const float inv = rsqrt_f32(d2);
+        const float inv2 = inv * inv;
+        const float qq = xi.w * xj.w;
+        const int tj = __float_as_int(aj.w);
+        const float2 lj = ljrow[tj];
+        const float r6 = inv2 * inv2 * inv2;
+        const float flj = __builtin_fmaf(lj.x, r6, -lj.y) * r6 * inv2;
+        const float fc = qq * inv2 * inv;
+        const float far = qq * a.cfar * inv2 * inv;
+        const float nearv = qq * __builtin_fmaf(aj.z, d2 * inv, aj.y);
+        float fdir = d2 > aj.x ? far : nearv;
+        bool excl = false;
+        if (near) {
+            const int w = j - i + 64;
+            if (w >= 0 && w < 128) excl = (ex[w >> 5] >> (w & 31)) & 1u;
+        }
+        if (a.excl_ptr) {  // fallback: exclusions beyond the 128-wide window
+            const int e0 = a.excl_ptr[i], e1 = a.excl_ptr[i + 1];
+            for (int e = e0; e < e1; ++e) excl |= (a.excl_idx[e] == j);
+        }
+        const bool self = (j == i);
+        const float fnb = (excl || self) ? 0.0f : flj + fc;
+        fdir = self ? 0.0f : fdir;
+        if (!SPLIT) {
+            const float s = fnb + fdir;
+            s0 = __builtin_fmaf(s, rx, s0);
+            s1 = __builtin_fmaf(s, ry, s1);
+            s2 = __builtin_fmaf(s, rz, s2);
+        } else {
+            const float fl = (excl || self) ? 0.0f : flj;
+            const float fcc = (excl || self) ? 0.0f : fc;
+            s0 = __builtin_fmaf(fl, rx, s0);
+            s1 = __builtin_fmaf(fl, ry, s1);
+            s2 = __builtin_fmaf(fl, rz, s2);
+            c0 = __builtin_fmaf(fcc, rx, c0);
+            c1 = __builtin_fmaf(fcc, ry, c1);
+            c2 = __builtin_fmaf(fcc, rz, c2);
+            p0 = __builtin_fmaf(fdir, rx, p0);
+            p1 = __builtin_fmaf(fdir, ry, p1);
+            p2 = __builtin_fmaf(fdir, rz, p2);
+        }
+    }
+    acc[0] += s0; acc[1] += s1; acc[2] += s2;
+    if (SPLIT) {
+        acc[3] += c0; acc[4] += c1; acc[5] += c2;
+        acc[6] += p0; acc[7] += p1; acc[8] += p2;
+    }
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(kBlk) void k_nonbond(NonbondArgs a)
+{
+    extern __shared__ float2 lds_lj[];
+    for (int t = threadIdx.x; t < a.T * a.T; t += kBlk) lds_lj[t] = a.lj[t];
+    __syncthreads();
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * a.atoms_per_chunk;
+    const int j1 = min(a.n, j0 + a.atoms_per_chunk);
+    const bool valid = i < a.n;
+    const int ii = valid ? i : a.n - 1;  // idle lanes mirror the last atom, results dropped
+    const float4 xi = a.xyzq[ii];
+    const int ti = __float_as_int(a.aux[ii].w);
+    uint32_t ex[4];
+    for (int k = 0; k < 4; ++k) ex[k] = a.excl[4 * ii + k];
+    const bool pbc = *a.pbc_flag != 0;
+    // wave bounds for the near-diagonal test
+    const int wave_lo = (blockIdx.x * kBlk + (threadIdx.x & ~(kWave - 1)));
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const float2 *ljrow = lds_lj + ti * a.T;
+    constexpr int kTile = 64;
+    for (int jt = j0; jt < j1; jt += kTile) {
+        const int je = min(j1, jt + kTile);
+        const bool near = (jt < wave_lo + kWave + 64) && (je > wave_lo - 64);
+        if (pbc)
+            nonbond_tile<true, SPLIT>(a, ii, jt, je, xi, ti, ex, near, ljrow, acc);
+        else
+            nonbond_tile<false, SPLIT>(a, ii, jt, je, xi, ti, ex, near, ljrow, acc);
+    }
+    if (!valid) return;
+    const size_t N = (size_t)a.n;
+    const int comps = SPLIT ? 9 : 3;
+    double *o = a.part + (size_t)chunk * comps * N;
+    for (int k = 0; k < comps; ++k) o[k * N + i] = acc[k];
+}
+
+hipError_t launch_nonbond(const NonbondArgs &a, int chunks, bool split, hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    const size_t lds = sizeof(float2) * a.T * a.T;
+    if (split)
+        hipLaunchKernelGGL(k_nonbond<true>, dim3(nblocks(a.n), chunks), dim3(kBlk), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_nonbond<false>, dim3(nblocks(a.n), chunks), dim3(kBlk), lds, s, a);
+    return hipGetLastError();
+}
+
+// sum [chunk][comps][N] partials over chunks in order; comps == 3: f += sum;
+// comps == 9: three FP64 outputs
+__global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const double *__restrict__ part, int n,
+                                                        int chunks, int comps, float4 *f,
+                                                        double *o0, double *o1, double *o2)
+{
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const size_t N = (size_t)n;
+    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < chunks; ++c)
+        for (int k = 0; k < comps; ++k) s[k] += part[((size_t)c * comps + k) * N + i];
+    if (comps == 3) {
+        float4 v = f[i];
+        v.x += (float)s[0];
+        v.y += (float)s[1];
+        v.z += (float)s[2];
+        f[i] = v;
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            o0[3 * i + k] = s[k];
+            o1[3 * i + k] = s[3 + k];
+            o2[3 * i + k] = s[6 + k];
+        }
+    }
+}
+
+hipError_t launch_chunk_reduce3(const double *part, int n, int chunks, int comps, float4 *f,
+                                double *o0, double *o1, double *o2, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_reduce3, dim3(nblocks(n)), dim3(kBlk), 0, s, part, n, chunks, comps,
+                       f, o0, o1, o2);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// bonded terms.  Each term writes its per-atom forces into fixed contribution
+// slots; k_bonded_gather sums each atom's slots in FP64 in a fixed order.
+// Slot layout: bonds [0, 2nb), angles [2nb, 2nb+3na), dihedrals 6 per term
+// (atoms i, j, k, l, then the 1-4 pair's i and l).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float3 mi3(float4 a, float4 b, float L, float h)
+{
+    return make_float3(min_image1(a.x - b.x, L, h), min_image1(a.y - b.y, L, h),
+                       min_image1(a.z - b.z, L, h));
+}
+
+// src/bond_force_cuda.cu:15-53
+__device__ void bond_term(const BondedArgs &a, int q)
+{
+    const int4 at = a.bond_atoms[q];
+    const float2 pr = a.bond_params[at.z];
+    const float3 r = mi3(a.xyzq[at.x], a.xyzq[at.y], a.L, a.h);
+    const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const float rinv = rsqrt_f32(d2);
+    const float fb = pr.x * (pr.y * rinv - 1.0f);
+    a.contrib[2 * q] = make_float4(fb * r.x, fb * r.y, fb * r.z, 0.0f);
+    a.contrib[2 * q + 1] = make_float4(-fb * r.x, -fb * r.y, -fb * r.z, 0.0f);
+    const float dd = d2 * rinv - pr.y;
+    a.energy[q] = 0.5 * (double)pr.x * (double)dd * (double)dd;
+}
+
+// src/angle_force_cuda.cu:12-64
+__device__ void angle_term(const BondedArgs &a, int q)
+{
+    const int4 at = a.angle_atoms[q];
+    const float3 r1 = mi3(a.xyzq[at.x], a.xyzq[at.y], a.L, a.h);
+    const float3 r2 = mi3(a.xyzq[at.y], a.xyzq[at.z], a.L, a.h);
+    const float c11 = r1.x * r1.x + r1.y * r1.y + r1.z * r1.z;
+    const float c22 = r2.x * r2.x + r2.y * r2.y + r2.z * r2.z;
+    const float c12 = r1.x * r2.x + r1.y * r2.y + r1.z * r2.z;
+    const float b = -c12 / sqrtf(c11 * c22);
+    float th;
+    if (b >= 1.0f) th = 1.0e-16f;
+    else if (b <= -1.0f) th = 3.1415926535f;
+    else th = acosf(b);
+    const float2 pr = a.angle_params[at.w];
+    const float fa = pr.x * (th - pr.y) / sqrtf(c11 * c22 - c12 * c12);
+    const float a11 = c12 / c11, a22 = c12 / c22;
+    const int base = 2 * a.nb + 3 * q;
+    a.contrib[base] = make_float4(fa * (a11 * r1.x - r2.x), fa * (a11 * r1.y - r2.y),
+                                  fa * (a11 * r1.z - r2.z), 0.0f);
+    a.contrib[base + 1] = make_float4(fa * ((1.0f + a22) * r2.x - (1.0f + a11) * r1.x),
+                                      fa * ((1.0f + a22) * r2.y - (1.0f + a11) * r1.y),
+                                      fa * ((1.0f + a22) * r2.z - (1.0f + a11) * r1.z), 0.0f);
+    a.contrib[base + 2] = make_float4(fa * (r1.x - a22 * r2.x), fa * (r1.y - a22 * r2.y),
+                                      fa * (r1.z - a22 * r2.z), 0.0f);
+    const double dth = (double)th - (double)pr.y;
+    a.energy[a.nb + q] = 0.5 * (double)pr.x * dth * dth;
+}
+
+// src/dih_force_cuda.cu:13-136
+__device__ void dih_term(const BondedArgs &a, int q)
+{
+    int4 at = a.dih_atoms[q];
+    const int ty = a.dih_types[q];
+    const float4 p1 = a.xyzq[at.x], p2 = a.xyzq[at.y];
+    const float4 p3 = a.xyzq[abs(at.z)], p4 = a.xyzq[abs(at.w)];
+    const int base = 2 * a.nb + 3 * a.na + 6 * q;
+    double e14c = 0.0, e14v = 0.0;
+    if (at.z > 0 && at.w > 0) {  // scaled 1-4 (index 0 never qualifies, quirk Q20 mirrored)
+        const float2 lj = a.lj[a.ityp[at.x] * a.T + a.ityp[at.w]];
+        const float2 sc = a.sc14[ty];
+        const float3 r = mi3(p1, p4, a.L, a.h);
+        const float rm = r.x * r.x + r.y * r.y + r.z * r.z;
+        const float r6 = 1.0f / (rm * rm * rm);
+        const float fe = p1.w * p4.w / (rm * sqrtf(rm) * sc.x);
+        const float fv = r6 * (12.0f * lj.x * r6 - 6.0f * lj.y) / (sc.y * rm);
+        const float s = fe + fv;
+        a.contrib[base + 4] = make_float4(s * r.x, s * r.y, s * r.z, 0.0f);
+        a.contrib[base + 5] = make_float4(-s * r.x, -s * r.y, -s * r.z, 0.0f);
+        e14c = (double)p1.w * (double)p4.w / (sqrt((double)rm) * (double)sc.x);
+        e14v = ((double)lj.x * r6 * r6 - (double)lj.y * r6) / (double)sc.y;
+    } else {
+        a.contrib[base + 4] = make_float4(0.f, 0.f, 0.f, 0.f);
+        a.contrib[base + 5] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const float3 r12 = mi3(p1, p2, a.L, a.h);
+    const float3 r23 = mi3(p2, p3, a.L, a.h);
+    const float3 r34 = mi3(p3, p4, a.L, a.h);
+    const float c11 = r12.x * r12.x + r12.y * r12.y + r12.z * r12.z;
+    const float c22 = r23.x * r23.x + r23.y * r23.y + r23.z * r23.z;
+    const float c12 = r12.x * r23.x + r12.y * r23.y + r12.z * r23.z;
+    const float c33 = r34.x * r34.x + r34.y * r34.y + r34.z * r34.z;
+    const float c23 = r23.x * r34.x + r23.y * r34.y + r23.z * r34.z;
+    const float c13 = r12.x * r34.x + r12.y * r34.y + r12.z * r34.z;
+    const float t1 = c13 * c22 - c12 * c23;
+    const float t2 = c11 * c23 - c12 * c13;
+    const float t3 = c12 * c12 - c11 * c22;
+    const float t4 = c22 * c33 - c23 * c23;
+    const float t5 = c13 * c23 - c12 * c33;
+    const float t6 = -t1;
+    const float b = sqrtf(-t3 * t4);
+    const float aa = t6 / b;
+    const float4 pr = a.dih_params[ty];
+    float fd = 0.0f, phi;
+    if (aa <= -1.0f) {
+        phi = 3.1415926535f;
+    } else if (aa >= 1.0f) {
+        phi = 0.0f;
+    } else {
+        phi = acosf(aa);
+        fd = pr.x * pr.y * sinf(pr.x * phi - pr.z) * c22 / (sinf(phi) * b);
+    }
+    const float g1 = fd / t3, g4 = -fd / t4;
+    const float3 f1 = make_float3(g1 * (t1 * r12.x + t2 * r23.x + t3 * r34.x),
+                                  g1 * (t1 * r12.y + t2 * r23.y + t3 * r34.y),
+                                  g1 * (t1 * r12.z + t2 * r23.z + t3 * r34.z));
+    const float3 f4 = make_float3(g4 * (t4 * r12.x + t5 * r23.x + t6 * r34.x),
+                                  g4 * (t4 * r12.y + t5 * r23.y + t6 * r34.y),
+                                  g4 * (t4 * r12.z + t5 * r23.z + t6 * r34.z));
+    const float a12 = c12 / c22, a23 = c23 / c22;
+    a.contrib[base] = make_float4(f1.x, f1.y, f1.z, 0.f);
+    a.contrib[base + 1] = make_float4(-(1.0f + a12) * f1.x + a23 * f4.x,
+                                      -(1.0f + a12) * f1.y + a23 * f4.y,
+                                      -(1.0f + a12) * f1.z + a23 * f4.z, 0.f);
+    a.contrib[base + 2] = make_float4(a12 * f1.x - (1.0f + a23) * f4.x,
+                                      a12 * f1.y - (1.0f + a23) * f4.y,
+                                      a12 * f1.z - (1.0f + a23) * f4.z, 0.f);
+    a.contrib[base + 3] = make_float4(f4.x, f4.y, f4.z, 0.f);
+    const size_t eb = (size_t)a.nb + a.na + 3 * (size_t)q;
+    a.energy[eb] = (double)pr.y * (1.0 + cos((double)pr.x * (double)phi - (double)pr.z));
+    a.energy[eb + 1] = e14c;
+    a.energy[eb + 2] = e14v;
+}
+
+__global__ __launch_bounds__(kBlk) void k_bonded_terms(BondedArgs a)
+{
+    const int q = blockIdx.x * kBlk + threadIdx.x;
+    if (q < a.nb) {
+        if (a.terms & 1) {
+            bond_term(a, q);
+        } else {
+            a.contrib[2 * q] = a.contrib[2 * q + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            a.energy[q] = 0.0;
+        }
+    } else if (q < a.nb + a.na) {
+        const int k = q - a.nb;
+        if (a.terms & 2) {
+            angle_term(a, k);
+        } else {
+            const int base = 2 * a.nb + 3 * k;
+            for (int s = 0; s < 3; ++s) a.contrib[base + s] = make_float4(0.f, 0.f, 0.f, 0.f);
+            a.energy[a.nb + k] = 0.0;
+        }
+    } else if (q < a.nb + a.na + a.nd) {
+        const int k = q - a.nb - a.na;
+        if (a.terms & 4) {
+            dih_term(a, k);
+        } else {
+            const int base = 2 * a.nb + 3 * a.na + 6 * k;
+            for (int s = 0; s < 6; ++s) a.contrib[base + s] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int s = 0; s < 3; ++s) a.energy[(size_t)a.nb + a.na + 3 * (size_t)k + s] = 0.0;
+        }
+    }
+}
+
+hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s)
+{
+    const long n = (long)a.nb + a.na + a.nd;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bonded_terms, dim3(nblocks(n)), dim3(kBlk), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlk) void k_bonded_gather(const float4 *__restrict__ contrib,
+                                                        const int *__restrict__ ptr,
+                                                        const int *__restrict__ idx, int n,
+                                                        float4 *f)
+{
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
+        const float4 c = contrib[idx[e]];
+        sx += c.x;
+        sy += c.y;
+        sz += c.z;
+    }
+    float4 v = f[i];
+    v.x += (float)sx;
+    v.y += (float)sy;
+    v.z += (float)sz;
+    f[i] = v;
+}
+
+hipError_t launch_bonded_gather(const float4 *contrib, const int *ptr, const int *idx, int n,
+                                float4 *f, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bonded_gather, dim3(nblocks(n)), dim3(kBlk), 0, s, contrib, ptr, idx, n,
+                       f);
+    return hipGetLastError();
+}
+
+// energies: bond, angle, torsion, 1-4 Coulomb, 1-4 LJ (one workgroup, fixed tree)
+__global__ __launch_bounds__(kBlk) void k_bonded_energy(const double *__restrict__ e, int nb,
+                                                        int na, int nd, double *out)
+{
+    __shared__ double red[5][kBlk];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int q = threadIdx.x; q < nb; q += kBlk) acc[0] += e[q];
+    for (int q = threadIdx.x; q < na; q += kBlk) acc[1] += e[nb + q];
+    for (int q = threadIdx.x; q < nd; q += kBlk)
+        for (int s = 0; s < 3; ++s) acc[2 + s] += e[(size_t)nb + na + 3 * (size_t)q + s];
+    for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) out[k] = red[k][0];
+}
+
+hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, double *out5,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bonded_energy, dim3(1), dim3(kBlk), 0, s, eterm, nb, na, nd, out5);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// A.15 leap-frog + Andersen (src/leapfrog_cuda.cu:33-67).  On a collision the
+// velocity is redrawn from Maxwell-Boltzmann, then half-kicked (quirk Q21).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlk) void k_leapfrog(int n, float4 *xyzq, float4 *vel,
+                                                   const float4 *__restrict__ f, float kT,
+                                                   float dt, uint32_t thr, float L, uint64_t seed,
+                                                   uint32_t sys, uint32_t step_arg,
+                                                   const uint32_t *__restrict__ step_d)
+{
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t step = step_d ? *step_d : step_arg;
+    const uint4 o = philox_draw(seed, (uint32_t)i, kStreamAndersen, 0u, step, sys);
+    float4 v = vel[i];
+    const float4 fi = f[i];
+    float4 x = xyzq[i];
+    const float m = v.w;
+    if (o.x < thr) {
+        const uint4 z = philox_draw(seed, (uint32_t)i, kStreamAndersen, 1u, step, sys);
+        const float s24 = 5.9604644775390625e-08f;
+        const float u1a = (float)((z.x >> 8) + 1u) * s24, u2a = (float)(z.y >> 8) * s24;
+        const float u1b = (float)((z.z >> 8) + 1u) * s24, u2b = (float)(z.w >> 8) * s24;
+        const float ra = sqrtf(-2.0f * logf(u1a)), rb = sqrtf(-2.0f * logf(u1b));
+        const float twopi = 6.283185307179586f;
+        const float sc = sqrtf(kT / m);
+        const float hk = dt / (m * 2.0f);
+        v.x = ra * cosf(twopi * u2a) * sc + fi.x * hk;
+        v.y = ra * sinf(twopi * u2a) * sc + fi.y * hk;
+        v.z = rb * cosf(twopi * u2b) * sc + fi.z * hk;
+    } else {
+        const float kk = dt / m;
+        v.x += fi.x * kk;
+        v.y += fi.y * kk;
+        v.z += fi.z * kk;
+    }
+    x.x += v.x * dt;
+    x.y += v.y * dt;
+    x.z += v.z * dt;
+    // wrap (src/cuda_vector_routines.h:49-68)
+    x.x = x.x > L ? x.x - L : (x.x < 0.0f ? x.x + L : x.x);
+    x.y = x.y > L ? x.y - L : (x.y < 0.0f ? x.y + L : x.y);
+    x.z = x.z > L ? x.z - L : (x.z < 0.0f ? x.z + L : x.z);
+    vel[i] = v;
+    xyzq[i] = x;
+}
+
+hipError_t launch_leapfrog(int n, float4 *xyzq, float4 *vel, const float4 *f, float kT, float dt,
+                           uint32_t thr, float L, uint64_t seed, uint32_t sys, uint32_t step,
+                           const uint32_t *step_d, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leapfrog, dim3(nblocks(n)), dim3(kBlk), 0, s, n, xyzq, vel, f, kT, dt,
+                       thr, L, seed, sys, step, step_d);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// A.16 umbrella restraint (src/us_force_cuda.cu:14-47).  One workgroup: the
+// two centres of mass by an FP64 tree reduction (the reference synchronises
+// only within a block, quirk Q12), then each bias atom's force.  Atoms in the
+// list are unique, so the force updates do not race.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlk) void k_us(int nbias, const int2 *__restrict__ atoms,
+                                             const float *__restrict__ mass, float k0, float k1,
+                                             float x0, float L, const float4 *__restrict__ xyzq,
+                                             float4 *f, double *com)
+{
+    __shared__ double red[6][kBlk];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nbias; b += kBlk) {
+        const int2 at = atoms[b];
+        const float4 x = xyzq[at.x];
+        const double m = mass[b];
+        acc[3 * at.y + 0] += m * x.x;
+        acc[3 * at.y + 1] += m * x.y;
+        acc[3 * at.y + 2] += m * x.z;
+    }
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    const double h = 0.5 * (double)L;
+    double dv[3];
+    for (int k = 0; k < 3; ++k) {
+        double r = red[k][0] - red[3 + k][0];
+        if (r > h) r -= L;
+        else if (r < -h) r += L;
+        dv[k] = r;
+    }
+    const double d = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+    for (int b = threadIdx.x; b < nbias; b += kBlk) {
+        const int2 at = atoms[b];
+        const double pre = (double)(at.y == 0 ? k0 : k1) * (d - (double)x0) * (double)mass[b] / d;
+        float4 v = f[at.x];
+        v.x += (float)(pre * dv[0]);
+        v.y += (float)(pre * dv[1]);
+        v.z += (float)(pre * dv[2]);
+        f[at.x] = v;
+    }
+    if (com && threadIdx.x < 6) com[threadIdx.x] = red[threadIdx.x][0];
+}
+
+hipError_t launch_us(int nbias, const int2 *atoms, const float *mass, float k0, float k1,
+                     float x0, float L, const float4 *xyzq, float4 *f, double *com,
+                     hipStream_t s)
+{
+    if (nbias == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_us, dim3(1), dim3(kBlk), 0, s, nbias, atoms, mass, k0, k1, x0, L, xyzq,
+                       f, com);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// utilities
+// ---------------------------------------------------------------------------
+// flag = 1 when some pair separation may exceed half a box (minimum image
+// needed); 0 when (extent + rmax + margin) < L/2 on every axis, in which case
+// the single-image minimum image is the identity for every atom-atom and
+// point-atom pair and the kernels take their shift-free loops.
+__global__ __launch_bounds__(1024) void k_pbc_flag(const float4 *__restrict__ xyzq, int n,
+                                                   float rmax_all, float L, int *flag)
+{
+    __shared__ float lo[3][1024], hi[3][1024];
+    float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, u[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const float4 x = xyzq[i];
+        l[0] = fminf(l[0], x.x); u[0] = fmaxf(u[0], x.x);
+        l[1] = fminf(l[1], x.y); u[1] = fmaxf(u[1], x.y);
+        l[2] = fminf(l[2], x.z); u[2] = fmaxf(u[2], x.z);
+    }
+    for (int k = 0; k < 3; ++k) {
+        lo[k][threadIdx.x] = l[k];
+        hi[k][threadIdx.x] = u[k];
+    }
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 3; ++k) {
+                lo[k][threadIdx.x] = fminf(lo[k][threadIdx.x], lo[k][threadIdx.x + w]);
+                hi[k][threadIdx.x] = fmaxf(hi[k][threadIdx.x], hi[k][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int need = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float ext = hi[k][0] - lo[k][0];
+            if (!(ext + rmax_all + 1.0e-2f < 0.5f * L)) need = 1;  // NaN -> need
+        }
+        *flag = need;
+    }
+}
+
+hipError_t launch_pbc_flag(const float4 *xyzq, int n, float rmax_all, float L, int *flag,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pbc_flag, dim3(1), dim3(1024), 0, s, xyzq, n, rmax_all, L, flag);
+    return hipGetLastError();
+}
+
+__global__ void k_zero16(uint4 *p, size_t n16)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n16;
+         k += (size_t)gridDim.x * blockDim.x)
+        p[k] = make_uint4(0u, 0u, 0u, 0u);
+}
+__global__ void k_zero1(unsigned char *p, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0;
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
+{
+    if (bytes == 0) return hipSuccess;
+    if (((uintptr_t)p & 15u) == 0 && (bytes & 15u) == 0) {
+        const size_t n16 = bytes / 16;
+        const int g = (int)std::min<size_t>(4096, (n16 + kBlk - 1) / kBlk);
+        hipLaunchKernelGGL(k_zero16, dim3(g), dim3(kBlk), 0, s, (uint4 *)p, n16);
+    } else {
+        const int g = (int)std::min<size_t>(4096, (bytes + kBlk - 1) / kBlk);
+        hipLaunchKernelGGL(k_zero1, dim3(g), dim3(kBlk), 0, s, (unsigned char *)p, bytes);
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_step_advance(uint32_t *step) { *step += 1u; }
+__global__ void k_set_word(uint32_t *dst, uint32_t v) { *dst = v; }
+
+hipError_t launch_step_advance(uint32_t *step_d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, s, step_d);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_word(uint32_t *dst, uint32_t v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, dst, v);
+    return hipGetLastError();
+}
+
+}  // namespace isspa
