@@ -49,6 +49,9 @@ int isspa_memcpy_dtoh(void *dst_h, const void *src_d, size_t bytes, void *stream
 int isspa_memcpy_dtod(void *dst_d, const void *src_d, size_t bytes, void *stream);
 /* zero-fill with a kernel (safe inside a captured graph) */
 int isspa_zero(void *dst_d, size_t bytes, void *stream);
+/* dst4_d[i].xyz += src4_d[i].xyz for i < n (float4 arrays); used to fold the
+ * separately kept IS-SPA force (the reference's isspaf_d) into for_d */
+int isspa_add4(float *dst4_d, const float *src4_d, int n, void *stream);
 int isspa_stream_create(void **stream);
 int isspa_stream_destroy(void *stream);
 int isspa_stream_sync(void *stream);

@@ -112,6 +112,13 @@ extern "C" int isspa_zero(void *dst, size_t bytes, void *stream)
     HIPCHK(launch_zero(dst, bytes, S(stream)));
     return ISSPA_OK;
 }
+extern "C" int isspa_add4(float *dst4_d, const float *src4_d, int n, void *stream)
+{
+    if (n < 0 || (n > 0 && (!dst4_d || !src4_d))) return fail(ISSPA_ERR_ARG, "isspa_add4: bad argument");
+    HIPCHK(launch_add4(reinterpret_cast<float4 *>(dst4_d), reinterpret_cast<const float4 *>(src4_d),
+                       n, S(stream)));
+    return ISSPA_OK;
+}
 extern "C" int isspa_stream_create(void **stream)
 {
     hipStream_t s;

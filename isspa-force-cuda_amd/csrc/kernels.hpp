@@ -139,6 +139,7 @@ hipError_t launch_us(int nbias, const int2 *atoms, const float *mass, float k0, 
 hipError_t launch_pbc_flag(const float4 *xyzq, int n, float rmax_all, float L, int *flag,
                            hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
+hipError_t launch_add4(float4 *dst, const float4 *src, int n, hipStream_t s);
 hipError_t launch_step_advance(uint32_t *step_d, hipStream_t s);
 hipError_t launch_set_word(uint32_t *dst, uint32_t v, hipStream_t s);
 

@@ -589,6 +589,25 @@ hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(kBlk) void k_add4(float4 *dst, const float4 *__restrict__ src, int n)
+{
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= n) return;
+    float4 d = dst[i];
+    const float4 s = src[i];
+    d.x += s.x;
+    d.y += s.y;
+    d.z += s.z;
+    dst[i] = d;
+}
+
+hipError_t launch_add4(float4 *dst, const float4 *src, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add4, dim3(nblocks(n)), dim3(kBlk), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
 __global__ void k_step_advance(uint32_t *step) { *step += 1u; }
 __global__ void k_set_word(uint32_t *dst, uint32_t v) { *dst = v; }
 

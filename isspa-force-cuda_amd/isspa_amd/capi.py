@@ -49,6 +49,7 @@ _SIGS = {
     "isspa_memcpy_dtoh": ([vp, vp, C.c_size_t, vp], i32),
     "isspa_memcpy_dtod": ([vp, vp, C.c_size_t, vp], i32),
     "isspa_zero": ([vp, C.c_size_t, vp], i32),
+    "isspa_add4": ([vp, vp, i32, vp], i32),
     "isspa_stream_create": ([C.POINTER(vp)], i32),
     "isspa_stream_destroy": ([vp], i32),
     "isspa_stream_sync": ([vp], i32),
