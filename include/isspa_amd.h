@@ -60,6 +60,23 @@ int isspa_timer_begin(void **timer, void *stream);
 int isspa_timer_end(void *timer, void *stream, float *ms);
 
 /* ------------------------------------------------------------------------- */
+/* Batched windows.  W independent systems (e.g. umbrella-sampling windows,  */
+/* each started by the reference as its own process from its own deck,       */
+/* src/total_force_cuda.cpp:36-100) share one model and one set of launches:  */
+/* their atoms are concatenated in window order, window w owning              */
+/* window_atoms[w] consecutive atoms.  Atoms interact only within their       */
+/* window.  system_ids[w] (NULL: w) is window w's Philox counter word 3 and   */
+/* lbox[w] (NULL: the model's lbox) its box.  A model created without windows */
+/* is a batch of one.  The reference has no batched form.                     */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int n_windows;
+    const int *window_atoms;     /* int[n_windows] */
+    const uint32_t *system_ids;  /* uint32[n_windows] or NULL */
+    const float *lbox;           /* float[n_windows] or NULL */
+} isspa_windows;
+
+/* ------------------------------------------------------------------------- */
 /* IS-SPA solvent model: class isspa's device state (src/isspa_class.h:11-68) */
 /* plus isspa_grid_block (src/isspa_force_cuda.cu:362-408).                    */
 /* ------------------------------------------------------------------------- */
@@ -90,6 +107,13 @@ int isspa_default_constants(float *e0, float *p0, float *q0, float *o0, float *e
  * the G-table bin). */
 int isspa_model_create(const isspa_model_desc *desc, const int *isspa_type, const float *rmax,
                        const float *g, const float *e, const float *f, isspa_model **out);
+/* The same over windows: desc->n_atoms is the total, isspa_type is indexed by
+ * the concatenated atoms, the tables are shared (concatenate the windows'
+ * types).  Every window's points, fields and forces are those of the window
+ * alone with system_id = system_ids[w]. */
+int isspa_model_create_batch(const isspa_model_desc *desc, const isspa_windows *windows,
+                             const int *isspa_type, const float *rmax, const float *g,
+                             const float *e, const float *f, isspa_model **out);
 int isspa_model_destroy(isspa_model *m);
 
 /* isspa_force_cuda (src/isspa_force_cuda.cu:331-360): MC points for `step`,
@@ -117,9 +141,12 @@ int isspa_model_get_field(isspa_model *m, float *pts4_d, double *g_d, int *cnt_d
  * positions xyzq4_d; leaves caller buffers untouched.  Synchronous. */
 int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int reps, float *ms, void *stream);
 /* Tuning knobs (0 = automatic): atom splits of the field pass, point chunks of
- * the force pass. */
+ * the force pass, applied to every window.  get_tiling reports the largest
+ * window's.  The automatic choice aims at `waves` wavefronts per pass
+ * (default 8192) shared among windows in proportion to their work. */
 int isspa_model_set_tiling(isspa_model *m, int field_splits, int force_chunks);
 int isspa_model_get_tiling(const isspa_model *m, int *field_splits, int *force_chunks);
+int isspa_model_set_target_waves(isspa_model *m, int waves);
 
 /* ------------------------------------------------------------------------- */
 /* All-pairs nonbond: nonbond_force_cuda + nonbond_force_cuda_grid_block      */
@@ -139,6 +166,13 @@ int nonbond_model_create(int n_atoms, int n_lj_types, const int *ityp, const int
                          const int *excl_list, int excl_len, const int *isspa_type,
                          int n_isspa_types, const float *rmax, float ep, float lbox,
                          nonbond_model **out);
+/* The same over windows (pairs only within a window; exclusions must not
+ * cross windows). */
+int nonbond_model_create_batch(int n_atoms, int n_lj_types, const int *ityp, const int *nbparm,
+                               const float *lj_a, const float *lj_b, const int *nexcl_prefix,
+                               const int *excl_list, int excl_len, const int *isspa_type,
+                               int n_isspa_types, const float *rmax, float ep, float lbox,
+                               const isspa_windows *windows, nonbond_model **out);
 int nonbond_model_destroy(nonbond_model *m);
 int nonbond_force(nonbond_model *m, const float *xyzq4_d, float *f4_d, void *stream);
 /* components in FP64 (double[N*3] each, overwritten): LJ, Coulomb, IS-SPA pair */
@@ -193,9 +227,16 @@ typedef struct us_model us_model;
  * normalised by the group mass, kumb float[2] = {-k, +k}, x0 the centre. */
 int us_model_create(int n_atoms, int nbias, const int *atom_list, const float *mass,
                     const float *kumb, float x0, float lbox, us_model **out);
+/* n_res restraints (one per umbrella window): restraint r owns bias entries
+ * [bias_offsets[r], bias_offsets[r+1]) of atom_list/mass (global atom indices),
+ * force constant k[r] (group 0 gets -k, group 1 +k), centre x0[r], box lbox[r]. */
+int us_model_create_batch(int n_atoms, int n_res, const int *bias_offsets, const int *atom_list,
+                          const float *mass, const float *k, const float *x0, const float *lbox,
+                          us_model **out);
 int us_model_destroy(us_model *m);
-/* f4_d += restraint forces; com_d, if not NULL, receives double[6] (COM of
- * group 0 then group 1; the CV is |min_image(com0 - com1)|, us::print_cv). */
+/* f4_d += restraint forces; com_d, if not NULL, receives double[6 n_res] (COM
+ * of group 0 then group 1 per restraint; the CV is |min_image(com0 - com1)|,
+ * us::print_cv). */
 int us_force(us_model *m, const float *xyzq4_d, float *f4_d, double *com_d, void *stream);
 
 /* ------------------------------------------------------------------------- */
@@ -203,7 +244,9 @@ int us_force(us_model *m, const float *xyzq4_d, float *f4_d, double *com_d, void
 /* ------------------------------------------------------------------------- */
 typedef struct md_engine md_engine;
 /* Components may be NULL (skipped).  The engine does not own them.  Buffers
- * are the caller's device arrays; the step counter lives in the engine. */
+ * are the caller's device arrays; the step counter lives in the engine.  The
+ * integrator uses the IS-SPA (else nonbond) model's windows: system_ids and
+ * boxes per window; lbox/system_id apply when neither model is given. */
 int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_model *bonded, us_model *us,
                      int n_atoms, float *xyzq4_d, float *vel4_d, float *f4_d, float kT,
                      float dt, float pnu, float lbox, uint64_t seed, uint32_t system_id,
@@ -218,6 +261,8 @@ int md_engine_capture(md_engine *e, int steps_per_graph, void *stream);
 int md_engine_replay(md_engine *e, int replays, void *stream);
 /* forces only (zero, bonded, IS-SPA, nonbond, US) for the current step */
 int md_engine_forces(md_engine *e, void *stream);
+/* the restraint centres of mass of the last force evaluation, double[6 n_res] */
+int md_engine_get_com(md_engine *e, double *com_d, void *stream);
 
 #ifdef __cplusplus
 }

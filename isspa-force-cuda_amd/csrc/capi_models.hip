@@ -1,192 +1,91 @@
-// C-ABI implementation (include/isspa_amd.h): model objects, device buffers,
-// launch geometry, and the hipGraph-captured MD step.
-#include <hip/hip_runtime.h>
-
+// C-ABI models (include/isspa_amd.h): IS-SPA, nonbond, bonded, umbrella,
+// integrator and the hipGraph-captured MD engine.  Every model is built over
+// a window layout: W independent systems with concatenated atoms; a single
+// system is W = 1.  Launch geometry (per-window splits/chunks and the 1-D
+// block work tables) is decided here on the host.
 #include <algorithm>
 #include <cmath>
-#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <new>
-#include <string>
+#include <numeric>
 #include <vector>
 
-#include "../../include/isspa_amd.h"
-#include "kernels.hpp"
+#include "capi_common.hpp"
 
 using namespace isspa;
 
-// ---------------------------------------------------------------------------
-// errors
-// ---------------------------------------------------------------------------
-static thread_local std::string g_err;
-
-static int fail(int code, const char *fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIPCHK(expr)                                                                    \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(ISSPA_ERR_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
-                        __FILE__, __LINE__);                                            \
-    } while (0)
-
-#define CHK(expr)                 \
-    do {                          \
-        int r_ = (expr);          \
-        if (r_ != ISSPA_OK) return r_; \
-    } while (0)
-
-extern "C" const char *isspa_last_error(void) { return g_err.c_str(); }
-extern "C" int isspa_abi_version(void) { return 1; }
-
-static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
-
-template <class T>
-static int dalloc(T **p, size_t count)
-{
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess)
-        return fail(ISSPA_ERR_ALLOC, "hipMalloc(%zu bytes): %s", count * sizeof(T),
-                    hipGetErrorString(e));
-    return ISSPA_OK;
-}
-
-template <class T>
-static int upload(T **p, const T *h, size_t count)
-{
-    CHK(dalloc(p, count));
-    if (count) HIPCHK(hipMemcpy(*p, h, count * sizeof(T), hipMemcpyHostToDevice));
-    return ISSPA_OK;
-}
-
-template <class T>
-static void dfree(T *&p)
-{
-    if (p) (void)hipFree((void *)p);
-    p = nullptr;
-}
+#define fail isspa_fail
 
 // ---------------------------------------------------------------------------
-// device memory / streams
+// window layout
 // ---------------------------------------------------------------------------
-extern "C" int isspa_dev_malloc(void **ptr_d, size_t bytes)
-{
-    if (!ptr_d) return fail(ISSPA_ERR_ARG, "isspa_dev_malloc: null out pointer");
-    return dalloc((unsigned char **)ptr_d, bytes);
-}
-extern "C" int isspa_dev_free(void *ptr_d)
-{
-    if (ptr_d) HIPCHK(hipFree(ptr_d));
-    return ISSPA_OK;
-}
-extern "C" int isspa_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream)
-{
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)));
-    return ISSPA_OK;
-}
-extern "C" int isspa_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream)
-{
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)));
-    HIPCHK(hipStreamSynchronize(S(stream)));
-    return ISSPA_OK;
-}
-extern "C" int isspa_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream)
-{
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
-    return ISSPA_OK;
-}
-extern "C" int isspa_zero(void *dst, size_t bytes, void *stream)
-{
-    HIPCHK(launch_zero(dst, bytes, S(stream)));
-    return ISSPA_OK;
-}
-extern "C" int isspa_add4(float *dst4_d, const float *src4_d, int n, void *stream)
-{
-    if (n < 0 || (n > 0 && (!dst4_d || !src4_d))) return fail(ISSPA_ERR_ARG, "isspa_add4: bad argument");
-    HIPCHK(launch_add4(reinterpret_cast<float4 *>(dst4_d), reinterpret_cast<const float4 *>(src4_d),
-                       n, S(stream)));
-    return ISSPA_OK;
-}
-extern "C" int isspa_stream_create(void **stream)
-{
-    hipStream_t s;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    *stream = s;
-    return ISSPA_OK;
-}
-extern "C" int isspa_stream_destroy(void *stream)
-{
-    if (stream) HIPCHK(hipStreamDestroy(S(stream)));
-    return ISSPA_OK;
-}
-extern "C" int isspa_stream_sync(void *stream)
-{
-    HIPCHK(hipStreamSynchronize(S(stream)));
-    return ISSPA_OK;
-}
-
-struct timer_pair {
-    hipEvent_t a, b;
+struct Windows {
+    int W = 0, ntot = 0;
+    std::vector<int> a0, n;
+    std::vector<uint32_t> sys;
+    std::vector<float> L;
+    std::vector<int> atom_win;  // host copy
 };
-extern "C" int isspa_timer_begin(void **timer, void *stream)
+
+static int build_windows(int ntot, const isspa_windows *win, uint32_t sys0, float L0, Windows &out)
 {
-    timer_pair *t = new (std::nothrow) timer_pair;
-    if (!t) return fail(ISSPA_ERR_ALLOC, "timer allocation");
-    HIPCHK(hipEventCreate(&t->a));
-    HIPCHK(hipEventCreate(&t->b));
-    HIPCHK(hipEventRecord(t->a, S(stream)));
-    *timer = t;
-    return ISSPA_OK;
-}
-extern "C" int isspa_timer_end(void *timer, void *stream, float *ms)
-{
-    timer_pair *t = (timer_pair *)timer;
-    if (!t) return fail(ISSPA_ERR_ARG, "null timer");
-    HIPCHK(hipEventRecord(t->b, S(stream)));
-    HIPCHK(hipEventSynchronize(t->b));
-    HIPCHK(hipEventElapsedTime(ms, t->a, t->b));
-    (void)hipEventDestroy(t->a);
-    (void)hipEventDestroy(t->b);
-    delete t;
+    out = Windows{};
+    out.ntot = ntot;
+    if (!win) {
+        out.W = 1;
+        out.a0 = {0};
+        out.n = {ntot};
+        out.sys = {sys0};
+        out.L = {L0};
+    } else {
+        if (win->n_windows < 1 || !win->window_atoms)
+            return fail(ISSPA_ERR_ARG, "windows: n_windows=%d, window_atoms %s", win->n_windows,
+                        win->window_atoms ? "set" : "NULL");
+        out.W = win->n_windows;
+        int acc = 0;
+        for (int w = 0; w < out.W; ++w) {
+            const int nw = win->window_atoms[w];
+            if (nw < 1) return fail(ISSPA_ERR_ARG, "windows: window %d has %d atoms", w, nw);
+            out.a0.push_back(acc);
+            out.n.push_back(nw);
+            out.sys.push_back(win->system_ids ? win->system_ids[w] : (uint32_t)w);
+            const float L = win->lbox ? win->lbox[w] : L0;
+            if (!(L > 0)) return fail(ISSPA_ERR_ARG, "windows: lbox of window %d is %g", w, L);
+            out.L.push_back(L);
+            acc += nw;
+        }
+        if (acc != ntot)
+            return fail(ISSPA_ERR_ARG, "windows: window atoms sum to %d, model has %d", acc, ntot);
+    }
+    out.atom_win.resize(ntot);
+    for (int w = 0; w < out.W; ++w)
+        for (int i = 0; i < out.n[w]; ++i) out.atom_win[out.a0[w] + i] = w;
     return ISSPA_OK;
 }
 
-extern "C" int isspa_default_constants(float *e0, float *p0, float *q0, float *o0, float *ep)
+static WinDesc base_desc(const Windows &L, int w)
 {
-    // src/constants.h:6-13, evaluated in FP32 (volatile keeps the compiler from
-    // folding in higher precision)
-    volatile float pi = 3.1415926535f, rho = 0.0075f, epv = 2.3473f;
-    volatile float p0v = 4.173705894203977f;
-    const float e0v = 3.0f / (4.0f * pi * rho * p0v) * (1.0f - 1.0f / epv);
-    if (e0) *e0 = e0v;
-    if (p0) *p0 = p0v;
-    if (q0) *q0 = 1.6899960102882601f;
-    if (o0) *o0 = 9.3658298730779350f;
-    if (ep) *ep = epv;
-    return ISSPA_OK;
+    WinDesc d{};
+    d.a0 = L.a0[w];
+    d.n = L.n[w];
+    d.sys = L.sys[w];
+    d.L = L.L[w];
+    d.h = 0.5f * L.L[w];
+    return d;
 }
 
-// launch geometry: enough waves to cover the 256 CUs several times over
-static constexpr long kTargetWaves = 8192;
+// enough waves to cover the 256 CUs several times over
+static constexpr long kDefaultTargetWaves = 8192;
 
-static int auto_splits(long lanes, long work, long min_work)
+// pieces for `lanes` lanes doing `work` items each, aiming at `target` waves,
+// with at least `min_work` items per piece
+static int auto_pieces(long long lanes, long long work, long long min_work, double target)
 {
-    const long waves = std::max(1L, (lanes + 63) / 64);
-    long s = (kTargetWaves + waves - 1) / waves;
-    s = std::min(s, std::max(1L, work / min_work));
-    return (int)std::max(1L, s);
+    const long long waves = std::max(1LL, (lanes + 63) / 64);
+    long long s = (long long)std::ceil(target / (double)waves);
+    s = std::min(s, std::max(1LL, work / min_work));
+    return (int)std::max(1LL, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -194,9 +93,10 @@ static int auto_splits(long lanes, long work, long min_work)
 // ---------------------------------------------------------------------------
 struct isspa_model {
     isspa_model_desc d{};
-    int P = 0;
+    Windows lay;
+    long long P = 0;
     float rmax_all = 0.f;
-    int *itype = nullptr;
+    int *itype = nullptr, *atom_win = nullptr;
     float *rmax = nullptr, *rmax2 = nullptr, *vtot = nullptr;
     float *g = nullptr, *e = nullptr, *f = nullptr;
     float4 *pts = nullptr;
@@ -204,71 +104,108 @@ struct isspa_model {
     double *g_out = nullptr, *E_out = nullptr, *E0_out = nullptr;
     int *cnt_out = nullptr;
     int *pbc_flag = nullptr;
-    // tiling and partial buffers
-    int splits = 0, chunks = 0, atoms_per_split = 0, pts_per_chunk = 0;
+    // tiling
     int user_splits = 0, user_chunks = 0;
+    long target_waves = kDefaultTargetWaves;
+    std::vector<WinDesc> wins_h;
+    WinDesc *wins = nullptr;
+    int4 *fblk = nullptr, *pblk = nullptr;
+    int nfblk = 0, npblk = 0;
     float *part_g = nullptr;
     int *part_cnt = nullptr;
     double *part_s0 = nullptr, *part_ein = nullptr;
     double *part_f = nullptr;
-    size_t cap_field = 0, cap_force = 0;
+    long long F = 0, C = 0, cap_field = 0, cap_force = 0;
     float4 *prof_f = nullptr;  // force sink of isspa_model_profile
     TableView tv{};
 };
 
-static int isspa_tile(isspa_model *m)
-{
-    const int n = m->d.n_atoms, P = m->P;
-    int s = m->user_splits > 0 ? m->user_splits : auto_splits(P, n, 64);
-    s = std::max(1, std::min(s, std::max(1, n)));
-    m->atoms_per_split = (n + s - 1) / s;
-    m->splits = (n + m->atoms_per_split - 1) / m->atoms_per_split;
-    int c = m->user_chunks > 0 ? m->user_chunks : auto_splits(n, P, 128);
-    c = std::max(1, std::min(c, std::max(1, P)));
-    m->pts_per_chunk = (P + c - 1) / c;
-    m->chunks = (P + m->pts_per_chunk - 1) / m->pts_per_chunk;
-    const size_t need_field = (size_t)m->splits * P;
-    if (need_field > m->cap_field) {
-        dfree(m->part_g);
-        dfree(m->part_cnt);
-        dfree(m->part_s0);
-        dfree(m->part_ein);
-        CHK(dalloc(&m->part_g, need_field));
-        CHK(dalloc(&m->part_cnt, need_field));
-        CHK(dalloc(&m->part_s0, 3 * need_field));
-        CHK(dalloc(&m->part_ein, 3 * need_field));
-        m->cap_field = need_field;
-    }
-    const size_t need_force = (size_t)m->chunks * 9 * n;
-    if (need_force > m->cap_force) {
-        dfree(m->part_f);
-        CHK(dalloc(&m->part_f, need_force));
-        m->cap_force = need_force;
-    }
-    return ISSPA_OK;
-}
-
 static void isspa_free(isspa_model *m)
 {
-    dfree(m->itype); dfree(m->rmax); dfree(m->rmax2); dfree(m->vtot);
+    dfree(m->itype); dfree(m->atom_win); dfree(m->rmax); dfree(m->rmax2); dfree(m->vtot);
     dfree(m->g); dfree(m->e); dfree(m->f);
     dfree(m->pts); dfree(m->rec); dfree(m->g_out); dfree(m->E_out); dfree(m->E0_out);
-    dfree(m->cnt_out); dfree(m->pbc_flag);
+    dfree(m->cnt_out); dfree(m->pbc_flag); dfree(m->wins); dfree(m->fblk); dfree(m->pblk);
     dfree(m->part_g); dfree(m->part_cnt); dfree(m->part_s0); dfree(m->part_ein); dfree(m->part_f);
     dfree(m->prof_f);
 }
 
-static int pad_table(const float *src, int T, int n, std::vector<float> &out)
+// per-window splits/chunks, partial offsets and the block work tables; blocks
+// are ordered by decreasing work so the longest ones start first
+static int isspa_tile(isspa_model *m)
+{
+    const Windows &L = m->lay;
+    const int nmc = m->d.n_mc;
+    double work_tot = 0;
+    for (int w = 0; w < L.W; ++w) work_tot += (double)L.n[w] * L.n[w] * nmc;
+    m->wins_h.assign(L.W, WinDesc{});
+    long long F = 0, C = 0;
+    std::vector<std::pair<long long, int4>> fb, pb;
+    for (int w = 0; w < L.W; ++w) {
+        WinDesc d = base_desc(L, w);
+        const long long n = L.n[w], P = n * nmc;
+        const double tgt = std::max(1.0, m->target_waves * ((double)n * P / work_tot));
+        int s = m->user_splits > 0 ? m->user_splits : auto_pieces(P, n, 32, tgt);
+        s = (int)std::max(1LL, std::min((long long)s, n));
+        d.aps = (int)((n + s - 1) / s);
+        d.splits = (int)((n + d.aps - 1) / d.aps);
+        int c = m->user_chunks > 0 ? m->user_chunks : auto_pieces(n, P, 128, tgt);
+        c = (int)std::max(1LL, std::min((long long)c, P));
+        d.ppc = (int)((P + c - 1) / c);
+        d.chunks = (int)((P + d.ppc - 1) / d.ppc);
+        d.fpo = F;
+        d.ppo = C;
+        F += (long long)d.splits * P;
+        C += (long long)d.chunks * n;
+        const int pblocks = (int)((P + 255) / 256), rblocks = (int)((n + 255) / 256);
+        for (int b = 0; b < pblocks; ++b)
+            for (int sp = 0; sp < d.splits; ++sp) fb.push_back({(long long)d.aps, make_int4(w, b, sp, 0)});
+        for (int b = 0; b < rblocks; ++b)
+            for (int ch = 0; ch < d.chunks; ++ch) pb.push_back({(long long)d.ppc, make_int4(w, b, ch, 0)});
+        m->wins_h[w] = d;
+    }
+    auto by_work = [](const std::pair<long long, int4> &a, const std::pair<long long, int4> &b) {
+        return a.first > b.first;
+    };
+    std::stable_sort(fb.begin(), fb.end(), by_work);
+    std::stable_sort(pb.begin(), pb.end(), by_work);
+    std::vector<int4> fbv, pbv;
+    for (auto &x : fb) fbv.push_back(x.second);
+    for (auto &x : pb) pbv.push_back(x.second);
+    dfree(m->wins); dfree(m->fblk); dfree(m->pblk);
+    CHK(upload(&m->wins, m->wins_h.data(), m->wins_h.size()));
+    CHK(upload(&m->fblk, fbv.data(), fbv.size()));
+    CHK(upload(&m->pblk, pbv.data(), pbv.size()));
+    m->nfblk = (int)fbv.size();
+    m->npblk = (int)pbv.size();
+    m->F = F;
+    m->C = C;
+    if (F > m->cap_field) {
+        dfree(m->part_g); dfree(m->part_cnt); dfree(m->part_s0); dfree(m->part_ein);
+        CHK(dalloc(&m->part_g, F));
+        CHK(dalloc(&m->part_cnt, F));
+        CHK(dalloc(&m->part_s0, 3 * F));
+        CHK(dalloc(&m->part_ein, 3 * F));
+        m->cap_field = F;
+    }
+    if (C > m->cap_force) {
+        dfree(m->part_f);
+        CHK(dalloc(&m->part_f, 9 * C));
+        m->cap_force = C;
+    }
+    return ISSPA_OK;
+}
+
+static void pad_table(const float *src, int T, int n, std::vector<float> &out)
 {
     out.assign((size_t)T * (n + 1), 0.0f);
     for (int t = 0; t < T; ++t)
         for (int b = 0; b < n; ++b) out[(size_t)t * (n + 1) + b] = src[(size_t)t * n + b];
-    return n + 1;
 }
 
-extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_type,
-                                  const float *rmax, const float *g, const float *e,
-                                  const float *f, isspa_model **out)
+extern "C" int isspa_model_create_batch(const isspa_model_desc *d, const isspa_windows *win,
+                                        const int *isspa_type, const float *rmax, const float *g,
+                                        const float *e, const float *f, isspa_model **out)
 {
     if (!d || !out || !isspa_type || !rmax || !g || !e || !f)
         return fail(ISSPA_ERR_ARG, "isspa_model_create: null argument");
@@ -276,7 +213,7 @@ extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_ty
     if (d->n_atoms < 1 || d->n_types < 1 || d->n_mc < 1)
         return fail(ISSPA_ERR_ARG, "isspa_model_create: n_atoms=%d n_types=%d n_mc=%d must be >= 1",
                     d->n_atoms, d->n_types, d->n_mc);
-    if ((long)d->n_atoms * d->n_mc > 0x7fffffffL)
+    if ((long long)d->n_atoms * d->n_mc > 0x7fffffffLL)
         return fail(ISSPA_ERR_ARG, "isspa_model_create: n_atoms*n_mc exceeds 2^31");
     if (d->n_g < 2 || d->n_e < d->n_g || d->n_f < 2)
         return fail(ISSPA_ERR_ARG, "isspa_model_create: table rows n_g=%d n_e=%d n_f=%d (need n_g>=2, n_e>=n_g, n_f>=2)",
@@ -287,18 +224,21 @@ extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_ty
         if (isspa_type[i] < 0 || isspa_type[i] >= d->n_types)
             return fail(ISSPA_ERR_ARG, "isspa_model_create: isspa_type[%d]=%d out of range [0,%d)",
                         i, isspa_type[i], d->n_types);
+    const int T = d->n_types;
+    for (int t = 0; t < T; ++t)
+        if (!(rmax[t] > 0)) return fail(ISSPA_ERR_ARG, "isspa_model_create: rmax[%d]=%g must be > 0", t, rmax[t]);
     isspa_model *m = new (std::nothrow) isspa_model;
     if (!m) return fail(ISSPA_ERR_ALLOC, "isspa_model_create: host allocation");
     m->d = *d;
-    m->P = d->n_atoms * d->n_mc;
-    const int T = d->n_types;
+    int rc = build_windows(d->n_atoms, win, d->system_id, d->lbox, m->lay);
+    if (rc != ISSPA_OK) {
+        delete m;
+        return rc;
+    }
+    m->P = (long long)d->n_atoms * d->n_mc;
     std::vector<float> r2(T), vt(T);
     const float pi = 3.1415926535f;  // src/constants.h:6
     for (int t = 0; t < T; ++t) {
-        if (!(rmax[t] > 0)) {
-            delete m;
-            return fail(ISSPA_ERR_ARG, "isspa_model_create: rmax[%d]=%g must be > 0", t, rmax[t]);
-        }
         r2[t] = rmax[t] * rmax[t];
         // src/isspa_class.cpp:110-112, double arithmetic stored as float
         vt[t] = (float)(4.0 / 3.0 * (double)pi * rmax[t] * rmax[t] * rmax[t] * 0.0074 /
@@ -306,10 +246,9 @@ extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_ty
         m->rmax_all = std::max(m->rmax_all, rmax[t]);
     }
     std::vector<float> gp, ep, fp;
-    int ldg = pad_table(g, T, d->n_g, gp);
-    int lde = pad_table(e, T, d->n_e, ep);
-    int ldf = pad_table(f, T, d->n_f, fp);
-    int rc = ISSPA_OK;
+    pad_table(g, T, d->n_g, gp);
+    pad_table(e, T, d->n_e, ep);
+    pad_table(f, T, d->n_f, fp);
 #define TRY(x)                      \
     if ((rc = (x)) != ISSPA_OK) {   \
         isspa_free(m);              \
@@ -317,6 +256,7 @@ extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_ty
         return rc;                  \
     }
     TRY(upload(&m->itype, isspa_type, d->n_atoms));
+    TRY(upload(&m->atom_win, m->lay.atom_win.data(), m->lay.atom_win.size()));
     TRY(upload(&m->rmax, rmax, T));
     TRY(upload(&m->rmax2, r2.data(), T));
     TRY(upload(&m->vtot, vt.data(), T));
@@ -329,18 +269,25 @@ extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_ty
     TRY(dalloc(&m->cnt_out, m->P));
     TRY(dalloc(&m->E_out, 3 * (size_t)m->P));
     TRY(dalloc(&m->E0_out, 3 * (size_t)m->P));
-    TRY(dalloc(&m->pbc_flag, 1));
+    TRY(dalloc(&m->pbc_flag, m->lay.W));
     TRY(isspa_tile(m));
 #undef TRY
     TableView &tv = m->tv;
     tv.g = m->g; tv.e = m->e; tv.f = m->f;
     tv.n_g = d->n_g; tv.n_e = d->n_e; tv.n_f = d->n_f;
-    tv.ld_g = ldg; tv.ld_e = lde; tv.ld_f = ldf;
+    tv.ld_g = d->n_g + 1; tv.ld_e = d->n_e + 1; tv.ld_f = d->n_f + 1;
     tv.g_r0 = d->g_r0; tv.g_dr = d->g_dr; tv.g_inv = 1.0f / d->g_dr;
     tv.e_r0 = d->e_r0; tv.e_dr = d->e_dr; tv.e_inv = 1.0f / d->e_dr;
     tv.f_r0 = d->f_r0; tv.f_dr = d->f_dr; tv.f_inv = 1.0f / d->f_dr;
     *out = m;
     return ISSPA_OK;
+}
+
+extern "C" int isspa_model_create(const isspa_model_desc *d, const int *isspa_type,
+                                  const float *rmax, const float *g, const float *e,
+                                  const float *f, isspa_model **out)
+{
+    return isspa_model_create_batch(d, nullptr, isspa_type, rmax, g, e, f, out);
 }
 
 extern "C" int isspa_model_destroy(isspa_model *m)
@@ -360,12 +307,64 @@ extern "C" int isspa_model_set_tiling(isspa_model *m, int field_splits, int forc
     return isspa_tile(m);
 }
 
+extern "C" int isspa_model_set_target_waves(isspa_model *m, int waves)
+{
+    if (!m || waves < 1) return fail(ISSPA_ERR_ARG, "isspa_model_set_target_waves: bad argument");
+    m->target_waves = waves;
+    return isspa_tile(m);
+}
+
 extern "C" int isspa_model_get_tiling(const isspa_model *m, int *field_splits, int *force_chunks)
 {
     if (!m) return fail(ISSPA_ERR_ARG, "null model");
-    if (field_splits) *field_splits = m->splits;
-    if (force_chunks) *force_chunks = m->chunks;
+    // the largest window's tiling (the only one for a single system)
+    int best = 0;
+    for (int w = 1; w < m->lay.W; ++w)
+        if (m->lay.n[w] > m->lay.n[best]) best = w;
+    if (field_splits) *field_splits = m->wins_h[best].splits;
+    if (force_chunks) *force_chunks = m->wins_h[best].chunks;
     return ISSPA_OK;
+}
+
+static FieldArgs field_args(isspa_model *m, const float4 *x)
+{
+    FieldArgs fa;
+    fa.wins = m->wins; fa.blocks = m->fblk; fa.pts = m->pts; fa.xyzq = x; fa.itype = m->itype;
+    fa.tab = m->tv; fa.nmc = m->d.n_mc; fa.e0 = m->d.e0; fa.pbc_flag = m->pbc_flag;
+    fa.part_g = m->part_g; fa.part_cnt = m->part_cnt; fa.part_s0 = m->part_s0;
+    fa.part_ein = m->part_ein; fa.F = m->F;
+    return fa;
+}
+
+static EpilogueArgs epilogue_args(isspa_model *m)
+{
+    EpilogueArgs ea;
+    ea.wins = m->wins; ea.atom_win = m->atom_win; ea.pts = m->pts; ea.Ptot = m->P;
+    ea.nmc = m->d.n_mc; ea.part_g = m->part_g; ea.part_cnt = m->part_cnt;
+    ea.part_s0 = m->part_s0; ea.part_ein = m->part_ein; ea.F = m->F;
+    ea.e0 = m->d.e0; ea.p0 = m->d.p0; ea.q0 = m->d.q0; ea.o0 = m->d.o0;
+    ea.rec = m->rec; ea.g_out = m->g_out; ea.cnt_out = m->cnt_out; ea.E_out = m->E_out;
+    ea.E0_out = m->E0_out;
+    return ea;
+}
+
+static ForceArgs force_args(isspa_model *m, const float4 *x)
+{
+    ForceArgs fa;
+    fa.wins = m->wins; fa.blocks = m->pblk; fa.rec4 = reinterpret_cast<const float4 *>(m->rec);
+    fa.xyzq = x; fa.itype = m->itype; fa.rmax2 = m->rmax2; fa.tab = m->tv; fa.nmc = m->d.n_mc;
+    fa.pbc_flag = m->pbc_flag; fa.part = m->part_f; fa.C = m->C;
+    return fa;
+}
+
+static ReduceArgs reduce_args(isspa_model *m, const float4 *x, float4 *f, double *flj,
+                              double *fpol, double *fcdd)
+{
+    ReduceArgs ra;
+    ra.wins = m->wins; ra.atom_win = m->atom_win; ra.part = m->part_f; ra.C = m->C;
+    ra.ntot = m->d.n_atoms; ra.xyzq = x; ra.itype = m->itype; ra.vtot = m->vtot; ra.f = f;
+    ra.f_lj = flj; ra.f_pol = fpol; ra.f_cdd = fcdd;
+    return ra;
 }
 
 // points + field + epilogue (shared by the total and component entry points)
@@ -373,25 +372,11 @@ static int isspa_field_pass(isspa_model *m, const float *xyzq4_d, uint32_t step,
                             const uint32_t *step_d, hipStream_t s)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
-    const isspa_model_desc &d = m->d;
-    HIPCHK(launch_pbc_flag(x, d.n_atoms, m->rmax_all, d.lbox, m->pbc_flag, s));
-    HIPCHK(launch_mc_points(x, m->itype, m->rmax, m->rmax2, d.n_atoms, d.n_mc, d.seed,
-                            d.system_id, step, step_d, m->pts, s));
-    FieldArgs fa;
-    fa.pts = m->pts; fa.xyzq = x; fa.itype = m->itype; fa.tab = m->tv;
-    fa.n = d.n_atoms; fa.P = m->P; fa.atoms_per_split = m->atoms_per_split;
-    fa.L = d.lbox; fa.h = 0.5f * d.lbox; fa.e0 = d.e0; fa.pbc_flag = m->pbc_flag;
-    fa.part_g = m->part_g; fa.part_cnt = m->part_cnt; fa.part_s0 = m->part_s0;
-    fa.part_ein = m->part_ein;
-    HIPCHK(launch_field(fa, m->splits, s));
-    EpilogueArgs ea;
-    ea.pts = m->pts; ea.P = m->P; ea.splits = m->splits;
-    ea.part_g = m->part_g; ea.part_cnt = m->part_cnt; ea.part_s0 = m->part_s0;
-    ea.part_ein = m->part_ein;
-    ea.e0 = d.e0; ea.p0 = d.p0; ea.q0 = d.q0; ea.o0 = d.o0;
-    ea.rec = m->rec; ea.g_out = m->g_out; ea.cnt_out = m->cnt_out; ea.E_out = m->E_out;
-    ea.E0_out = m->E0_out;
-    HIPCHK(launch_epilogue(ea, s));
+    HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, m->rmax_all, m->pbc_flag, s));
+    HIPCHK(launch_mc_points(m->wins, m->atom_win, x, m->itype, m->rmax, m->rmax2, m->d.n_atoms,
+                            m->d.n_mc, m->d.seed, step, step_d, m->pts, s));
+    HIPCHK(launch_field(field_args(m, x), m->nfblk, s));
+    HIPCHK(launch_epilogue(epilogue_args(m), s));
     return ISSPA_OK;
 }
 
@@ -399,18 +384,8 @@ static int isspa_force_pass(isspa_model *m, const float *xyzq4_d, bool split, fl
                             double *flj, double *fpol, double *fcdd, hipStream_t s)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
-    const isspa_model_desc &d = m->d;
-    ForceArgs fa;
-    fa.rec4 = reinterpret_cast<const float4 *>(m->rec);
-    fa.xyzq = x; fa.itype = m->itype; fa.rmax2 = m->rmax2; fa.tab = m->tv;
-    fa.n = d.n_atoms; fa.P = m->P; fa.pts_per_chunk = m->pts_per_chunk;
-    fa.L = d.lbox; fa.h = 0.5f * d.lbox; fa.pbc_flag = m->pbc_flag; fa.part = m->part_f;
-    HIPCHK(launch_force(fa, m->chunks, split, s));
-    ReduceArgs ra;
-    ra.part = m->part_f; ra.n = d.n_atoms; ra.chunks = m->chunks; ra.xyzq = x;
-    ra.itype = m->itype; ra.vtot = m->vtot; ra.f = f; ra.f_lj = flj; ra.f_pol = fpol;
-    ra.f_cdd = fcdd;
-    HIPCHK(launch_force_reduce(ra, split, s));
+    HIPCHK(launch_force(force_args(m, x), m->npblk, split, s));
+    HIPCHK(launch_force_reduce(reduce_args(m, x, f, flj, fpol, fcdd), split, s));
     return ISSPA_OK;
 }
 
@@ -439,60 +414,37 @@ extern "C" int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int rep
     if (!m || !xyzq4_d || !ms || reps < 1) return fail(ISSPA_ERR_ARG, "isspa_model_profile: bad argument");
     hipStream_t s = S(stream);
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
-    const isspa_model_desc &d = m->d;
-    if (!m->prof_f) CHK(dalloc(&m->prof_f, (size_t)d.n_atoms));
+    if (!m->prof_f) CHK(dalloc(&m->prof_f, (size_t)m->d.n_atoms));
     hipEvent_t ev[2];
     HIPCHK(hipEventCreate(&ev[0]));
     HIPCHK(hipEventCreate(&ev[1]));
-    HIPCHK(launch_pbc_flag(x, d.n_atoms, m->rmax_all, d.lbox, m->pbc_flag, s));
     int rc = ISSPA_OK;
-    for (int stage = 0; stage < 5 && rc == ISSPA_OK; ++stage) {
+    hipError_t e = launch_pbc_flag(m->wins, m->lay.W, x, m->rmax_all, m->pbc_flag, s);
+    for (int stage = 0; stage < 5 && rc == ISSPA_OK && e == hipSuccess; ++stage) {
         // one untimed launch, then `reps` timed back-to-back launches
-        for (int r = 0; r <= reps && rc == ISSPA_OK; ++r) {
-            if (r == 1) HIPCHK(hipEventRecord(ev[0], s));
-            hipError_t e = hipSuccess;
-            if (stage == 0) {
-                e = launch_mc_points(x, m->itype, m->rmax, m->rmax2, d.n_atoms, d.n_mc, d.seed,
-                                     d.system_id, 0u, nullptr, m->pts, s);
-            } else if (stage == 1 || stage == 2) {
-                FieldArgs fa;
-                fa.pts = m->pts; fa.xyzq = x; fa.itype = m->itype; fa.tab = m->tv;
-                fa.n = d.n_atoms; fa.P = m->P; fa.atoms_per_split = m->atoms_per_split;
-                fa.L = d.lbox; fa.h = 0.5f * d.lbox; fa.e0 = d.e0; fa.pbc_flag = m->pbc_flag;
-                fa.part_g = m->part_g; fa.part_cnt = m->part_cnt; fa.part_s0 = m->part_s0;
-                fa.part_ein = m->part_ein;
-                EpilogueArgs ea;
-                ea.pts = m->pts; ea.P = m->P; ea.splits = m->splits;
-                ea.part_g = m->part_g; ea.part_cnt = m->part_cnt; ea.part_s0 = m->part_s0;
-                ea.part_ein = m->part_ein;
-                ea.e0 = d.e0; ea.p0 = d.p0; ea.q0 = d.q0; ea.o0 = d.o0;
-                ea.rec = m->rec; ea.g_out = m->g_out; ea.cnt_out = m->cnt_out;
-                ea.E_out = m->E_out; ea.E0_out = m->E0_out;
-                e = stage == 1 ? launch_field(fa, m->splits, s) : launch_epilogue(ea, s);
-            } else {
-                ForceArgs fa;
-                fa.rec4 = reinterpret_cast<const float4 *>(m->rec);
-                fa.xyzq = x; fa.itype = m->itype; fa.rmax2 = m->rmax2; fa.tab = m->tv;
-                fa.n = d.n_atoms; fa.P = m->P; fa.pts_per_chunk = m->pts_per_chunk;
-                fa.L = d.lbox; fa.h = 0.5f * d.lbox; fa.pbc_flag = m->pbc_flag;
-                fa.part = m->part_f;
-                ReduceArgs ra;
-                ra.part = m->part_f; ra.n = d.n_atoms; ra.chunks = m->chunks; ra.xyzq = x;
-                ra.itype = m->itype; ra.vtot = m->vtot; ra.f = m->prof_f; ra.f_lj = nullptr;
-                ra.f_pol = nullptr; ra.f_cdd = nullptr;
-                e = stage == 3 ? launch_force(fa, m->chunks, false, s)
-                               : launch_force_reduce(ra, false, s);
+        for (int r = 0; r <= reps && e == hipSuccess; ++r) {
+            if (r == 1) e = hipEventRecord(ev[0], s);
+            if (e != hipSuccess) break;
+            switch (stage) {
+            case 0:
+                e = launch_mc_points(m->wins, m->atom_win, x, m->itype, m->rmax, m->rmax2,
+                                     m->d.n_atoms, m->d.n_mc, m->d.seed, 0u, nullptr, m->pts, s);
+                break;
+            case 1: e = launch_field(field_args(m, x), m->nfblk, s); break;
+            case 2: e = launch_epilogue(epilogue_args(m), s); break;
+            case 3: e = launch_force(force_args(m, x), m->npblk, false, s); break;
+            default:
+                e = launch_force_reduce(reduce_args(m, x, m->prof_f, nullptr, nullptr, nullptr),
+                                        false, s);
             }
-            if (e != hipSuccess)
-                rc = fail(ISSPA_ERR_HIP, "isspa_model_profile stage %d: %s", stage, hipGetErrorString(e));
         }
-        if (rc != ISSPA_OK) break;
-        HIPCHK(hipEventRecord(ev[1], s));
-        HIPCHK(hipEventSynchronize(ev[1]));
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
         float t = 0.f;
-        HIPCHK(hipEventElapsedTime(&t, ev[0], ev[1]));
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
         ms[stage] = t / (float)reps;
     }
+    if (e != hipSuccess) rc = fail(ISSPA_ERR_HIP, "isspa_model_profile: %s", hipGetErrorString(e));
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     return rc;
@@ -516,27 +468,33 @@ extern "C" int isspa_model_get_field(isspa_model *m, float *pts4_d, double *g_d,
 // nonbond model
 // ---------------------------------------------------------------------------
 struct nonbond_model {
+    Windows lay;
     int n = 0, T = 0;
-    float lbox = 0.f, cfar = 0.f;
+    float cfar = 0.f;
     float4 *aux = nullptr;
     float2 *lj = nullptr;
     uint32_t *excl = nullptr;
     int *excl_ptr = nullptr, *excl_idx = nullptr;
-    int *pbc_flag = nullptr;
-    int chunks = 0, atoms_per_chunk = 0;
+    int *pbc_flag = nullptr, *atom_win = nullptr;
+    std::vector<WinDesc> wins_h;
+    WinDesc *wins = nullptr;
+    int4 *blk = nullptr;
+    int nblk = 0;
+    long long B = 0;
     double *part = nullptr;
 };
 
 static void nonbond_free(nonbond_model *m)
 {
     dfree(m->aux); dfree(m->lj); dfree(m->excl); dfree(m->excl_ptr); dfree(m->excl_idx);
-    dfree(m->pbc_flag); dfree(m->part);
+    dfree(m->pbc_flag); dfree(m->atom_win); dfree(m->wins); dfree(m->blk); dfree(m->part);
 }
 
-extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nbparm,
-                                    const float *lj_a, const float *lj_b, const int *pre,
-                                    const int *list, int L, const int *isspa_type, int Ti,
-                                    const float *rmax, float ep, float lbox, nonbond_model **out)
+extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const int *nbparm,
+                                          const float *lj_a, const float *lj_b, const int *pre,
+                                          const int *list, int L, const int *isspa_type, int Ti,
+                                          const float *rmax, float ep, float lbox,
+                                          const isspa_windows *win, nonbond_model **out)
 {
     if (!out || !ityp || !nbparm || !lj_a || !lj_b || !pre || (!list && L > 0) || !isspa_type || !rmax)
         return fail(ISSPA_ERR_ARG, "nonbond_model_create: null argument");
@@ -558,7 +516,13 @@ extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nb
             return fail(ISSPA_ERR_ARG, "nbparm[%d]=%d out of range [0,%d)", k, nbparm[k], ntri);
     nonbond_model *m = new (std::nothrow) nonbond_model;
     if (!m) return fail(ISSPA_ERR_ALLOC, "host allocation");
-    m->n = n; m->T = T; m->lbox = lbox;
+    int rc = build_windows(n, win, 0u, lbox, m->lay);
+    if (rc != ISSPA_OK) {
+        delete m;
+        return rc;
+    }
+    m->n = n;
+    m->T = T;
     const double fep = 1.0 - 1.0 / (double)ep;
     m->cfar = (float)(-2.0 * fep / 3.0);
     std::vector<float4> aux(n);
@@ -580,8 +544,6 @@ extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nb
     std::vector<std::vector<int>> ex(n);
     for (int lo = 0; lo < n; ++lo) {
         const int start = lo == 0 ? 0 : pre[lo - 1], stop = pre[lo];
-        // every hi > lo that the scan would find: entries in ascending walk
-        // order up to the first entry larger than the partner
         for (int e = start; e < stop; ++e) {
             const int hi = list[e];
             if (hi <= lo || hi >= n) continue;
@@ -590,13 +552,17 @@ extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nb
                 if (list[k] == hi) { found = true; break; }
                 if (list[k] > hi) break;
             }
-            if (found) {
-                ex[lo].push_back(hi);
-                ex[hi].push_back(lo);
+            if (!found) continue;
+            if (m->lay.atom_win[lo] != m->lay.atom_win[hi]) {
+                nonbond_free(m);
+                delete m;
+                return fail(ISSPA_ERR_ARG, "exclusion %d-%d crosses windows", lo, hi);
             }
+            ex[lo].push_back(hi);
+            ex[hi].push_back(lo);
         }
     }
-    std::vector<uint32_t> win((size_t)4 * n, 0u);
+    std::vector<uint32_t> winmask((size_t)4 * n, 0u);
     std::vector<int> fptr(n + 1, 0), fidx;
     bool fallback = false;
     for (int i = 0; i < n; ++i) {
@@ -604,17 +570,44 @@ extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nb
         ex[i].erase(std::unique(ex[i].begin(), ex[i].end()), ex[i].end());
         for (int j : ex[i]) {
             const int w = j - i + 64;
-            if (w >= 0 && w < 128) win[4 * (size_t)i + (w >> 5)] |= 1u << (w & 31);
+            if (w >= 0 && w < 128) winmask[4 * (size_t)i + (w >> 5)] |= 1u << (w & 31);
             else fallback = true;
         }
     }
     if (fallback)
         for (int i = 0; i < n; ++i) {
             for (int j : ex[i])
-                if (std::abs(j - i) >= 64) fidx.push_back(j);
+                if (j - i + 64 < 0 || j - i + 64 >= 128) fidx.push_back(j);
             fptr[i + 1] = (int)fidx.size();
         }
-    int rc;
+    // partner chunks per window, work-balanced over the launch
+    const Windows &Ly = m->lay;
+    double work_tot = 0;
+    for (int w = 0; w < Ly.W; ++w) work_tot += (double)Ly.n[w] * Ly.n[w];
+    m->wins_h.assign(Ly.W, WinDesc{});
+    std::vector<std::pair<long long, int4>> bl;
+    long long B = 0;
+    for (int w = 0; w < Ly.W; ++w) {
+        WinDesc d = base_desc(Ly, w);
+        const long long nw = Ly.n[w];
+        const double tgt = std::max(1.0, kDefaultTargetWaves * ((double)nw * nw / work_tot));
+        const int c = auto_pieces(nw, nw, 64, tgt);
+        d.apc = (int)((nw + c - 1) / c);
+        d.nbc = (int)((nw + d.apc - 1) / d.apc);
+        d.npo = B;
+        B += (long long)d.nbc * nw;
+        for (int b = 0; b < (int)((nw + 255) / 256); ++b)
+            for (int ch = 0; ch < d.nbc; ++ch) bl.push_back({(long long)d.apc, make_int4(w, b, ch, 0)});
+        m->wins_h[w] = d;
+    }
+    std::stable_sort(bl.begin(), bl.end(),
+                     [](const std::pair<long long, int4> &a, const std::pair<long long, int4> &b) {
+                         return a.first > b.first;
+                     });
+    std::vector<int4> blv;
+    for (auto &x : bl) blv.push_back(x.second);
+    m->nblk = (int)blv.size();
+    m->B = B;
 #define TRY(x)                    \
     if ((rc = (x)) != ISSPA_OK) { \
         nonbond_free(m);          \
@@ -623,19 +616,28 @@ extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nb
     }
     TRY(upload(&m->aux, aux.data(), aux.size()));
     TRY(upload(&m->lj, lj.data(), lj.size()));
-    TRY(upload(&m->excl, win.data(), win.size()));
+    TRY(upload(&m->excl, winmask.data(), winmask.size()));
     if (fallback) {
         TRY(upload(&m->excl_ptr, fptr.data(), fptr.size()));
         TRY(upload(&m->excl_idx, fidx.data(), std::max<size_t>(1, fidx.size())));
     }
-    TRY(dalloc(&m->pbc_flag, 1));
-    int c = auto_splits(n, n, 64);
-    m->atoms_per_chunk = (n + c - 1) / c;
-    m->chunks = (n + m->atoms_per_chunk - 1) / m->atoms_per_chunk;
-    TRY(dalloc(&m->part, (size_t)m->chunks * 9 * n));
+    TRY(dalloc(&m->pbc_flag, Ly.W));
+    TRY(upload(&m->atom_win, Ly.atom_win.data(), Ly.atom_win.size()));
+    TRY(upload(&m->wins, m->wins_h.data(), m->wins_h.size()));
+    TRY(upload(&m->blk, blv.data(), blv.size()));
+    TRY(dalloc(&m->part, (size_t)9 * B));
 #undef TRY
     *out = m;
     return ISSPA_OK;
+}
+
+extern "C" int nonbond_model_create(int n, int T, const int *ityp, const int *nbparm,
+                                    const float *lj_a, const float *lj_b, const int *pre,
+                                    const int *list, int L, const int *isspa_type, int Ti,
+                                    const float *rmax, float ep, float lbox, nonbond_model **out)
+{
+    return nonbond_model_create_batch(n, T, ityp, nbparm, lj_a, lj_b, pre, list, L, isspa_type,
+                                      Ti, rmax, ep, lbox, nullptr, out);
 }
 
 extern "C" int nonbond_model_destroy(nonbond_model *m)
@@ -650,14 +652,14 @@ static int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float
                        double *o1, double *o2, hipStream_t s)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
-    HIPCHK(launch_pbc_flag(x, m->n, 0.0f, m->lbox, m->pbc_flag, s));
+    HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
     NonbondArgs a;
-    a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T; a.n = m->n;
-    a.atoms_per_chunk = m->atoms_per_chunk; a.excl = m->excl; a.excl_ptr = m->excl_ptr;
-    a.excl_idx = m->excl_idx; a.L = m->lbox; a.h = 0.5f * m->lbox; a.cfar = m->cfar;
-    a.pbc_flag = m->pbc_flag; a.part = m->part;
-    HIPCHK(launch_nonbond(a, m->chunks, split, s));
-    HIPCHK(launch_chunk_reduce3(m->part, m->n, m->chunks, split ? 9 : 3, f, o0, o1, o2, s));
+    a.wins = m->wins; a.blocks = m->blk; a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T;
+    a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
+    a.pbc_flag = m->pbc_flag; a.part = m->part; a.B = m->B;
+    HIPCHK(launch_nonbond(a, m->nblk, split, s));
+    HIPCHK(launch_chunk_reduce3(m->wins, m->atom_win, m->part, m->B, m->n, split ? 9 : 3, f, o0,
+                                o1, o2, s));
     return ISSPA_OK;
 }
 
@@ -677,7 +679,7 @@ extern "C" int nonbond_force_components(nonbond_model *m, const float *xyzq4_d, 
 }
 
 // ---------------------------------------------------------------------------
-// bonded model
+// bonded model (window-agnostic: terms carry global atom indices)
 // ---------------------------------------------------------------------------
 struct bonded_model {
     int n = 0, nb = 0, na = 0, nd = 0, T = 0;
@@ -848,50 +850,75 @@ extern "C" int leapfrog_andersen(int n, float *xyzq4_d, float *vel4_d, const flo
                                  uint32_t system_id, uint32_t step, const uint32_t *step_d,
                                  void *stream)
 {
-    if (n < 0 || !xyzq4_d || !vel4_d || !f4_d) return fail(ISSPA_ERR_ARG, "leapfrog_andersen: bad argument");
+    if (n < 0 || !xyzq4_d || !vel4_d || !f4_d || !(lbox > 0))
+        return fail(ISSPA_ERR_ARG, "leapfrog_andersen: bad argument");
     if (!(pnu >= 0.0f && pnu <= 1.0f)) return fail(ISSPA_ERR_ARG, "leapfrog_andersen: pnu=%g not in [0,1]", pnu);
-    HIPCHK(launch_leapfrog(n, reinterpret_cast<float4 *>(xyzq4_d), reinterpret_cast<float4 *>(vel4_d),
-                           reinterpret_cast<const float4 *>(f4_d), kT, dt, andersen_threshold(pnu),
-                           lbox, seed, system_id, step, step_d, S(stream)));
+    WinDesc one{};
+    one.n = n;
+    one.sys = system_id;
+    one.L = lbox;
+    one.h = 0.5f * lbox;
+    HIPCHK(launch_leapfrog(nullptr, nullptr, one, n, reinterpret_cast<float4 *>(xyzq4_d),
+                           reinterpret_cast<float4 *>(vel4_d), reinterpret_cast<const float4 *>(f4_d),
+                           kT, dt, andersen_threshold(pnu), seed, step, step_d, S(stream)));
     return ISSPA_OK;
 }
 
 // ---------------------------------------------------------------------------
-// umbrella restraint
+// umbrella restraints
 // ---------------------------------------------------------------------------
 struct us_model {
-    int nbias = 0;
+    int nres = 0, nbias = 0;
+    int *boff = nullptr;
     int2 *atoms = nullptr;
     float *mass = nullptr;
-    float k0 = 0.f, k1 = 0.f, x0 = 0.f, lbox = 0.f;
+    float2 *kx = nullptr;
+    float *L = nullptr;
 };
 
-extern "C" int us_model_create(int n_atoms, int nbias, const int *atom_list, const float *mass,
-                               const float *kumb, float x0, float lbox, us_model **out)
+static void us_free(us_model *m)
 {
-    if (!out || nbias < 1 || !atom_list || !mass || !kumb || !(lbox > 0))
+    dfree(m->boff); dfree(m->atoms); dfree(m->mass); dfree(m->kx); dfree(m->L);
+}
+
+extern "C" int us_model_create_batch(int n_atoms, int nres, const int *boff, const int *atom_list,
+                                     const float *mass, const float *k, const float *x0,
+                                     const float *lbox, us_model **out)
+{
+    if (!out || nres < 1 || !boff || !atom_list || !mass || !k || !x0 || !lbox)
         return fail(ISSPA_ERR_ARG, "us_model_create: bad argument");
     *out = nullptr;
-    std::vector<int> seen;
-    int ng[2] = {0, 0};
-    for (int b = 0; b < nbias; ++b) {
-        const int a = atom_list[2 * b], g = atom_list[2 * b + 1];
-        if (a < 0 || a >= n_atoms || (g != 0 && g != 1))
-            return fail(ISSPA_ERR_ARG, "us_model_create: entry %d = {%d,%d} invalid", b, a, g);
-        seen.push_back(a);
-        ng[g]++;
+    if (boff[0] != 0) return fail(ISSPA_ERR_ARG, "us_model_create: bias offsets must start at 0");
+    for (int r = 0; r < nres; ++r) {
+        if (boff[r + 1] <= boff[r]) return fail(ISSPA_ERR_ARG, "us_model_create: restraint %d is empty", r);
+        if (!(lbox[r] > 0)) return fail(ISSPA_ERR_ARG, "us_model_create: lbox of restraint %d", r);
+        std::vector<int> seen;
+        int ng[2] = {0, 0};
+        for (int b = boff[r]; b < boff[r + 1]; ++b) {
+            const int a = atom_list[2 * b], g = atom_list[2 * b + 1];
+            if (a < 0 || a >= n_atoms || (g != 0 && g != 1))
+                return fail(ISSPA_ERR_ARG, "us_model_create: entry %d = {%d,%d} invalid", b, a, g);
+            seen.push_back(a);
+            ng[g]++;
+        }
+        std::sort(seen.begin(), seen.end());
+        if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+            return fail(ISSPA_ERR_ARG, "us_model_create: an atom appears twice in restraint %d", r);
+        if (!ng[0] || !ng[1]) return fail(ISSPA_ERR_ARG, "us_model_create: restraint %d needs both groups", r);
     }
-    std::sort(seen.begin(), seen.end());
-    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
-        return fail(ISSPA_ERR_ARG, "us_model_create: an atom appears twice in the bias groups");
-    if (!ng[0] || !ng[1]) return fail(ISSPA_ERR_ARG, "us_model_create: both groups need atoms");
     us_model *m = new (std::nothrow) us_model;
     if (!m) return fail(ISSPA_ERR_ALLOC, "host allocation");
-    m->nbias = nbias; m->k0 = kumb[0]; m->k1 = kumb[1]; m->x0 = x0; m->lbox = lbox;
+    m->nres = nres;
+    m->nbias = boff[nres];
+    std::vector<float2> kx(nres);
+    for (int r = 0; r < nres; ++r) kx[r] = make_float2(k[r], x0[r]);
     int rc;
-    if ((rc = upload(&m->atoms, reinterpret_cast<const int2 *>(atom_list), nbias)) != ISSPA_OK ||
-        (rc = upload(&m->mass, mass, nbias)) != ISSPA_OK) {
-        dfree(m->atoms); dfree(m->mass);
+    if ((rc = upload(&m->boff, boff, (size_t)nres + 1)) != ISSPA_OK ||
+        (rc = upload(&m->atoms, reinterpret_cast<const int2 *>(atom_list), m->nbias)) != ISSPA_OK ||
+        (rc = upload(&m->mass, mass, m->nbias)) != ISSPA_OK ||
+        (rc = upload(&m->kx, kx.data(), kx.size())) != ISSPA_OK ||
+        (rc = upload(&m->L, lbox, nres)) != ISSPA_OK) {
+        us_free(m);
         delete m;
         return rc;
     }
@@ -899,11 +926,20 @@ extern "C" int us_model_create(int n_atoms, int nbias, const int *atom_list, con
     return ISSPA_OK;
 }
 
+extern "C" int us_model_create(int n_atoms, int nbias, const int *atom_list, const float *mass,
+                               const float *kumb, float x0, float lbox, us_model **out)
+{
+    if (!kumb || !out) return fail(ISSPA_ERR_ARG, "us_model_create: bad argument");
+    if (kumb[0] != -kumb[1])
+        return fail(ISSPA_ERR_NOTSUP, "us_model_create: kumb must be {-k, +k} (us::populate_mass)");
+    const int boff[2] = {0, nbias};
+    return us_model_create_batch(n_atoms, 1, boff, atom_list, mass, &kumb[1], &x0, &lbox, out);
+}
+
 extern "C" int us_model_destroy(us_model *m)
 {
     if (!m) return ISSPA_OK;
-    dfree(m->atoms);
-    dfree(m->mass);
+    us_free(m);
     delete m;
     return ISSPA_OK;
 }
@@ -912,7 +948,7 @@ extern "C" int us_force(us_model *m, const float *xyzq4_d, float *f4_d, double *
                         void *stream)
 {
     if (!m || !xyzq4_d || !f4_d) return fail(ISSPA_ERR_ARG, "us_force: null argument");
-    HIPCHK(launch_us(m->nbias, m->atoms, m->mass, m->k0, m->k1, m->x0, m->lbox,
+    HIPCHK(launch_us(m->nres, m->boff, m->atoms, m->mass, m->kx, m->L,
                      reinterpret_cast<const float4 *>(xyzq4_d), reinterpret_cast<float4 *>(f4_d),
                      com_d, S(stream)));
     return ISSPA_OK;
@@ -932,11 +968,22 @@ struct md_engine {
     float kT = 0.f, dt = 0.f, pnu = 0.f, lbox = 0.f;
     uint64_t seed = 0;
     uint32_t sys = 0;
+    WinDesc *wins = nullptr;   // integrator windows (owned)
+    int *atom_win = nullptr;
     uint32_t *step_d = nullptr;
     double *com_d = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+static void engine_free(md_engine *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    dfree(e->step_d); dfree(e->com_d); dfree(e->wins); dfree(e->atom_win);
+}
 
 extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_model *bonded,
                                 us_model *us, int n, float *x, float *v, float *f, float kT,
@@ -948,22 +995,33 @@ extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_mo
     if (nb && nb->n != n) return fail(ISSPA_ERR_ARG, "md_engine_create: nonbond model atom count mismatch");
     if (bonded && bonded->n != n) return fail(ISSPA_ERR_ARG, "md_engine_create: bonded model atom count mismatch");
     if (!(pnu >= 0.0f && pnu <= 1.0f)) return fail(ISSPA_ERR_ARG, "md_engine_create: pnu not in [0,1]");
+    if (isspa && nb && isspa->lay.W != nb->lay.W)
+        return fail(ISSPA_ERR_ARG, "md_engine_create: IS-SPA and nonbond window counts differ");
     md_engine *e = new (std::nothrow) md_engine;
     if (!e) return fail(ISSPA_ERR_ALLOC, "host allocation");
     e->isspa = isspa; e->nb = nb; e->bonded = bonded; e->us = us; e->n = n;
     e->x = x; e->v = v; e->f = f; e->kT = kT; e->dt = dt; e->pnu = pnu; e->lbox = lbox;
     e->seed = seed; e->sys = sys;
-    int rc;
-    if ((rc = dalloc(&e->step_d, 1)) != ISSPA_OK || (rc = dalloc(&e->com_d, 6)) != ISSPA_OK) {
-        dfree(e->step_d);
+    // integrator windows: those of the IS-SPA (else nonbond) model, else one
+    Windows lay;
+    int rc = ISSPA_OK;
+    if (isspa) lay = isspa->lay;
+    else if (nb) lay = nb->lay;
+    else rc = build_windows(n, nullptr, sys, lbox, lay);
+    std::vector<WinDesc> wd;
+    for (int w = 0; rc == ISSPA_OK && w < lay.W; ++w) wd.push_back(base_desc(lay, w));
+    if (rc == ISSPA_OK) rc = upload(&e->wins, wd.data(), wd.size());
+    if (rc == ISSPA_OK) rc = upload(&e->atom_win, lay.atom_win.data(), lay.atom_win.size());
+    if (rc == ISSPA_OK) rc = dalloc(&e->step_d, 1);
+    if (rc == ISSPA_OK) rc = dalloc(&e->com_d, 6 * (size_t)(us ? us->nres : 1));
+    if (rc == ISSPA_OK) {
+        hipError_t he = hipMemset(e->step_d, 0, sizeof(uint32_t));
+        if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
+    }
+    if (rc != ISSPA_OK) {
+        engine_free(e);
         delete e;
         return rc;
-    }
-    hipError_t he = hipMemset(e->step_d, 0, sizeof(uint32_t));
-    if (he != hipSuccess) {
-        dfree(e->step_d); dfree(e->com_d);
-        delete e;
-        return fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     }
     *out = e;
     return ISSPA_OK;
@@ -972,10 +1030,7 @@ extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_mo
 extern "C" int md_engine_destroy(md_engine *e)
 {
     if (!e) return ISSPA_OK;
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    dfree(e->step_d);
-    dfree(e->com_d);
+    engine_free(e);
     delete e;
     return ISSPA_OK;
 }
@@ -1004,9 +1059,9 @@ static int engine_forces(md_engine *e, hipStream_t s)
 static int engine_step(md_engine *e, hipStream_t s)
 {
     CHK(engine_forces(e, s));
-    HIPCHK(launch_leapfrog(e->n, reinterpret_cast<float4 *>(e->x), reinterpret_cast<float4 *>(e->v),
-                           reinterpret_cast<const float4 *>(e->f), e->kT, e->dt,
-                           andersen_threshold(e->pnu), e->lbox, e->seed, e->sys, 0u, e->step_d, s));
+    HIPCHK(launch_leapfrog(e->wins, e->atom_win, WinDesc{}, e->n, reinterpret_cast<float4 *>(e->x),
+                           reinterpret_cast<float4 *>(e->v), reinterpret_cast<const float4 *>(e->f),
+                           e->kT, e->dt, andersen_threshold(e->pnu), e->seed, 0u, e->step_d, s));
     HIPCHK(launch_step_advance(e->step_d, s));
     return ISSPA_OK;
 }
@@ -1021,6 +1076,14 @@ extern "C" int md_engine_step(md_engine *e, void *stream)
 {
     if (!e) return fail(ISSPA_ERR_ARG, "null engine");
     return engine_step(e, S(stream));
+}
+
+extern "C" int md_engine_get_com(md_engine *e, double *com_d, void *stream)
+{
+    if (!e || !com_d) return fail(ISSPA_ERR_ARG, "md_engine_get_com: bad argument");
+    const size_t k = 6 * (size_t)(e->us ? e->us->nres : 1);
+    HIPCHK(hipMemcpyAsync(com_d, e->com_d, k * sizeof(double), hipMemcpyDeviceToDevice, S(stream)));
+    return ISSPA_OK;
 }
 
 extern "C" int md_engine_capture(md_engine *e, int steps_per_graph, void *stream)

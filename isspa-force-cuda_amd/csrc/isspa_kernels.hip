@@ -18,6 +18,7 @@
 //  * no float atomics: lanes keep FP32 partials over short tiles, flush them
 //    into FP64, and write per-chunk partials that a reduction kernel sums in a
 //    fixed order (quirk Q10).
+//  * every launch covers any number of independent windows (kernels.hpp).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -39,21 +40,25 @@ __device__ __forceinline__ float point_coord(float x, float R, int v)
 
 // ---------------------------------------------------------------------------
 // A.1 MC points (src/isspa_force_cuda.cu:73-107), Philox instead of curand.
-// .w of each point carries the owner's rmax^2 for the field pass.
+// Point m of window w (owner m / nMC) is drawn with counter
+// {m, draw, step, sys_w}.  .w of each point carries the owner's rmax^2.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_mc_points(
+    const WinDesc *__restrict__ wins, const int *__restrict__ atom_win,
     const float4 *__restrict__ xyzq, const int *__restrict__ itype, const float *__restrict__ rmax,
-    const float *__restrict__ rmax2, int n, int nmc, uint64_t seed, uint32_t sys,
-    uint32_t step_arg, const uint32_t *__restrict__ step_d, float4 *__restrict__ pts)
+    const float *__restrict__ rmax2, int ntot, int nmc, uint64_t seed, uint32_t step_arg,
+    const uint32_t *__restrict__ step_d, float4 *__restrict__ pts)
 {
-    const int m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= n * nmc) return;
+    const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= (long long)ntot * nmc) return;
     const uint32_t step = step_d ? *step_d : step_arg;
-    const int k = m / nmc;
+    const int k = (int)(gp / nmc);
+    const WinDesc W = wins[atom_win[k]];
+    const uint32_t m = (uint32_t)(gp - (long long)W.a0 * nmc);
     const int t = itype[k];
     int v0 = 0, v1 = 0, v2 = 0;
     for (int a = 0; a < kMcMaxAttempts; ++a) {
-        const uint4 o = philox_draw(seed, (uint32_t)m, kStreamPoints, (uint32_t)a, step, sys);
+        const uint4 o = philox_draw(seed, m, kStreamPoints, (uint32_t)a, step, W.sys);
         const int c0 = 2 * (int)(o.x >> 8) - (1 << 24);
         const int c1 = 2 * (int)(o.y >> 8) - (1 << 24);
         const int c2 = 2 * (int)(o.z >> 8) - (1 << 24);
@@ -70,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_mc_points(
     p.y = point_coord(x.y, R, v1);
     p.z = point_coord(x.z, R, v2);
     p.w = rmax2[t];
-    pts[m] = p;
+    pts[gp] = p;
 }
 
 // ---------------------------------------------------------------------------
@@ -109,9 +114,9 @@ __device__ __forceinline__ void field_in_sphere(const TableView &tb, int tj, flo
 }
 
 template <bool PBC>
-__device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, float px, float py,
-                                           float pz, float R2, float &ax, float &ay, float &az,
-                                           FieldRare &st)
+__device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, float L, float h,
+                                           float px, float py, float pz, float R2, float &ax,
+                                           float &ay, float &az, FieldRare &st)
 {
     const float4 *__restrict__ xq = a.xyzq;
 #pragma unroll 4
@@ -119,9 +124,9 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, f
         const float4 q = xq[j];  // wave-uniform
         float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
         if (PBC) {
-            rx = min_image1(rx, a.L, a.h);
-            ry = min_image1(ry, a.L, a.h);
-            rz = min_image1(rz, a.L, a.h);
+            rx = min_image1(rx, L, h);
+            ry = min_image1(ry, L, h);
+            rz = min_image1(rz, L, h);
         }
         const float d2 = dist2_f32(rx, ry, rz);
         const float inv = rsqrt_f32(d2);
@@ -137,38 +142,40 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, f
 
 __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
 {
-    const int m = blockIdx.x * kBlock + threadIdx.x;
-    const int split = blockIdx.y;
-    const int j0 = split * a.atoms_per_split;
-    const int j1 = min(a.n, j0 + a.atoms_per_split);
-    const bool valid = m < a.P;
-    const float4 p = valid ? a.pts[m] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    const bool pbc = *a.pbc_flag != 0;
+    const int4 bt = a.blocks[blockIdx.x];
+    const WinDesc W = a.wins[bt.x];
+    const int P = W.n * a.nmc;
+    const int m = bt.y * kBlock + threadIdx.x;  // point within the window
+    const int split = bt.z;
+    const int j0 = W.a0 + split * W.aps;
+    const int j1 = min(W.a0 + W.n, j0 + W.aps);
+    const bool valid = m < P;
+    const long long gp = (long long)W.a0 * a.nmc + m;
+    const float4 p = valid ? a.pts[gp] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    const bool pbc = a.pbc_flag[bt.x] != 0;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     FieldRare st{1.0f, 0, 0.0, 0.0, 0.0};
     for (int jt = j0; jt < j1; jt += kFieldTile) {
         const int je = min(j1, jt + kFieldTile);
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         if (pbc)
-            field_tile<true>(a, jt, je, p.x, p.y, p.z, p.w, ax, ay, az, st);
+            field_tile<true>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
         else
-            field_tile<false>(a, jt, je, p.x, p.y, p.z, p.w, ax, ay, az, st);
+            field_tile<false>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
         sx += (double)ax;
         sy += (double)ay;
         sz += (double)az;
     }
     if (!valid) return;
-    const size_t P = (size_t)a.P;
-    a.part_g[split * P + m] = st.g;
-    a.part_cnt[split * P + m] = st.cnt;
-    double *s0 = a.part_s0 + (size_t)split * 3 * P;
-    double *ei = a.part_ein + (size_t)split * 3 * P;
-    s0[m] = sx;
-    s0[P + m] = sy;
-    s0[2 * P + m] = sz;
-    ei[m] = st.ex;
-    ei[P + m] = st.ey;
-    ei[2 * P + m] = st.ez;
+    const long long o = W.fpo + (long long)split * P + m;
+    a.part_g[o] = st.g;
+    a.part_cnt[o] = st.cnt;
+    a.part_s0[o] = sx;
+    a.part_s0[a.F + o] = sy;
+    a.part_s0[2 * a.F + o] = sz;
+    a.part_ein[o] = st.ex;
+    a.part_ein[a.F + o] = st.ey;
+    a.part_ein[2 * a.F + o] = st.ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -207,17 +214,20 @@ __device__ __forceinline__ void langevin_moments(double x, double &m1, double &m
 
 __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
 {
-    const int m = blockIdx.x * kBlock + threadIdx.x;
-    if (m >= a.P) return;
-    const size_t P = (size_t)a.P;
+    const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= a.Ptot) return;
+    const WinDesc W = a.wins[a.atom_win[gp / a.nmc]];
+    const int P = W.n * a.nmc;
+    const long long m = gp - (long long)W.a0 * a.nmc;
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
     int cnt = 0;
-    for (int s = 0; s < a.splits; ++s) {
-        g *= (double)a.part_g[s * P + m];
-        cnt += a.part_cnt[s * P + m];
+    for (int s = 0; s < W.splits; ++s) {
+        const long long o = W.fpo + (long long)s * P + m;
+        g *= (double)a.part_g[o];
+        cnt += a.part_cnt[o];
         for (int c = 0; c < 3; ++c) {
-            s0[c] += a.part_s0[(s * 3 + c) * P + m];
-            ei[c] += a.part_ein[(s * 3 + c) * P + m];
+            s0[c] += a.part_s0[c * a.F + o];
+            ei[c] += a.part_ein[c * a.F + o];
         }
     }
     double E0[3], E[3];
@@ -237,21 +247,20 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     const double A1 = (double)a.p0 * m1 * w;
     const double A2 = (double)a.q0 * m2 * w;
     const double A3 = (double)a.o0 * m3 * w;
-    const float4 p = a.pts[m];
-    float *r = a.rec + (size_t)m * kRecFloats;
-    float4 *r4 = reinterpret_cast<float4 *>(r);
+    const float4 p = a.pts[gp];
+    float4 *r4 = reinterpret_cast<float4 *>(a.rec + gp * kRecFloats);
     r4[0] = make_float4(p.x, p.y, p.z, (float)eh[0]);
     r4[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + cp * E0[0] / 3.0),
                         (float)(A1 * eh[1] + cp * E0[1] / 3.0));
     r4[2] = make_float4((float)(A1 * eh[2] + cp * E0[2] / 3.0), (float)(cp * E0[0]),
                         (float)(cp * E0[1]), (float)(cp * E0[2]));
     r4[3] = make_float4((float)A1, (float)A2, (float)A3, (float)w);
-    if (a.g_out) a.g_out[m] = g;
-    if (a.cnt_out) a.cnt_out[m] = cnt;
+    if (a.g_out) a.g_out[gp] = g;
+    if (a.cnt_out) a.cnt_out[gp] = cnt;
     if (a.E_out)
-        for (int c = 0; c < 3; ++c) a.E_out[3 * m + c] = E[c];
+        for (int c = 0; c < 3; ++c) a.E_out[3 * gp + c] = E[c];
     if (a.E0_out)
-        for (int c = 0; c < 3; ++c) a.E0_out[3 * m + c] = E0[c];
+        for (int c = 0; c < 3; ++c) a.E0_out[3 * gp + c] = E0[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -274,13 +283,14 @@ struct ForceAcc {
 };
 
 template <bool PBC, bool SPLIT>
-__device__ __forceinline__ void force_tile(const ForceArgs &a, int m0, int m1, float xi, float yi,
-                                           float zi, float R2, const float *__restrict__ Ftab,
-                                           float &sx, float &sy, float &sz, ForceAcc &acc)
+__device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, long long m1,
+                                           float L, float h, float xi, float yi, float zi,
+                                           float R2, const float *__restrict__ Ftab, float &sx,
+                                           float &sy, float &sz, ForceAcc &acc)
 {
     const float4 *__restrict__ rec = a.rec4;
 #pragma unroll 2
-    for (int m = m0; m < m1; ++m) {
+    for (long long m = m0; m < m1; ++m) {
         const float4 q0 = rec[4 * m + 0];  // wave-uniform
         const float4 q1 = rec[4 * m + 1];
         const float4 q2 = rec[4 * m + 2];
@@ -291,9 +301,9 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, int m0, int m1, f
         const float A1 = q3.x, A2 = q3.y, A3 = q3.z, w = q3.w;
         float rx = q0.x - xi, ry = q0.y - yi, rz = q0.z - zi;
         if (PBC) {
-            rx = min_image1(rx, a.L, a.h);
-            ry = min_image1(ry, a.L, a.h);
-            rz = min_image1(rz, a.L, a.h);
+            rx = min_image1(rx, L, h);
+            ry = min_image1(ry, L, h);
+            rz = min_image1(rz, L, h);
         }
         const float d2 = dist2_f32(rx, ry, rz);
         const float rinv = rsqrt_f32(d2);
@@ -363,42 +373,45 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, int m0, int m1, f
 template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int chunk = blockIdx.y;
-    const int m0 = chunk * a.pts_per_chunk;
-    const int m1 = min(a.P, m0 + a.pts_per_chunk);
-    const bool valid = i < a.n;
-    const float4 xi = valid ? a.xyzq[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const int ti = valid ? a.itype[i] : 0;
+    const int4 bt = a.blocks[blockIdx.x];
+    const WinDesc W = a.wins[bt.x];
+    const int il = bt.y * kBlock + threadIdx.x;  // receiver within the window
+    const int chunk = bt.z;
+    const long long pbase = (long long)W.a0 * a.nmc;
+    const long long m0 = pbase + (long long)chunk * W.ppc;
+    const long long m1 = min(pbase + (long long)W.n * a.nmc, m0 + W.ppc);
+    const bool valid = il < W.n;
+    const int i = W.a0 + (valid ? il : 0);
+    const float4 xi = a.xyzq[i];
+    const int ti = a.itype[i];
     const float R2 = valid ? a.rmax2[ti] : -1.0f;
     const float *Ftab = a.tab.f + ti * a.tab.ld_f;
-    const bool pbc = *a.pbc_flag != 0;
+    const bool pbc = a.pbc_flag[bt.x] != 0;
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int mt = m0; mt < m1; mt += kForceTile) {
-        const int me = min(m1, mt + kForceTile);
+    for (long long mt = m0; mt < m1; mt += kForceTile) {
+        const long long me = min(m1, mt + kForceTile);
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         if (pbc)
-            force_tile<true, SPLIT>(a, mt, me, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
+            force_tile<true, SPLIT>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
         else
-            force_tile<false, SPLIT>(a, mt, me, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
+            force_tile<false, SPLIT>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
         acc.ex += (double)sx;
         acc.ey += (double)sy;
         acc.ez += (double)sz;
     }
     if (!valid) return;
-    const size_t N = (size_t)a.n;
-    const int comps = SPLIT ? 9 : 6;
-    double *o = a.part + (size_t)chunk * comps * N;
-    o[0 * N + i] = acc.ex;
-    o[1 * N + i] = acc.ey;
-    o[2 * N + i] = acc.ez;
-    o[3 * N + i] = acc.lx;
-    o[4 * N + i] = acc.ly;
-    o[5 * N + i] = acc.lz;
+    const long long o = W.ppo + (long long)chunk * W.n + il;
+    const long long C = a.C;
+    a.part[o] = acc.ex;
+    a.part[C + o] = acc.ey;
+    a.part[2 * C + o] = acc.ez;
+    a.part[3 * C + o] = acc.lx;
+    a.part[4 * C + o] = acc.ly;
+    a.part[5 * C + o] = acc.lz;
     if (SPLIT) {
-        o[6 * N + i] = acc.cx;
-        o[7 * N + i] = acc.cy;
-        o[8 * N + i] = acc.cz;
+        a.part[6 * C + o] = acc.cx;
+        a.part[7 * C + o] = acc.cy;
+        a.part[8 * C + o] = acc.cz;
     }
 }
 
@@ -407,13 +420,14 @@ template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const size_t N = (size_t)a.n;
+    if (i >= a.ntot) return;
+    const WinDesc W = a.wins[a.atom_win[i]];
+    const int il = i - W.a0;
     const int comps = SPLIT ? 9 : 6;
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < a.chunks; ++c) {
-        const double *p = a.part + (size_t)c * comps * N;
-        for (int k = 0; k < comps; ++k) s[k] += p[k * N + i];
+    for (int c = 0; c < W.chunks; ++c) {
+        const long long o = W.ppo + (long long)c * W.n + il;
+        for (int k = 0; k < comps; ++k) s[k] += a.part[k * a.C + o];
     }
     const float4 x = a.xyzq[i];
     const double vt = (double)a.vtot[a.itype[i]];
@@ -436,50 +450,51 @@ __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
+static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-hipError_t launch_mc_points(const float4 *xyzq, const int *itype, const float *rmax,
-                            const float *rmax2, int n, int nmc, uint64_t seed, uint32_t sys,
-                            uint32_t step, const uint32_t *step_d, float4 *pts, hipStream_t s)
+hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const float4 *xyzq,
+                            const int *itype, const float *rmax, const float *rmax2, int ntot,
+                            int nmc, uint64_t seed, uint32_t step, const uint32_t *step_d,
+                            float4 *pts, hipStream_t s)
 {
-    const long P = (long)n * nmc;
+    const long long P = (long long)ntot * nmc;
     if (P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P)), dim3(kBlock), 0, s, xyzq, itype, rmax,
-                       rmax2, n, nmc, seed, sys, step, step_d, pts);
+    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P)), dim3(kBlock), 0, s, wins, atom_win, xyzq,
+                       itype, rmax, rmax2, ntot, nmc, seed, step, step_d, pts);
     return hipGetLastError();
 }
 
-hipError_t launch_field(const FieldArgs &a, int splits, hipStream_t s)
+hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s)
 {
-    if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_field, dim3(blocks_for(a.P), splits), dim3(kBlock), 0, s, a);
+    if (nblocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field, dim3(nblocks), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s)
 {
-    if (a.P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_field_epilogue, dim3(blocks_for(a.P)), dim3(kBlock), 0, s, a);
+    if (a.Ptot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_field_epilogue, dim3(blocks_for(a.Ptot)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_force(const ForceArgs &a, int chunks, bool split, hipStream_t s)
+hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s)
 {
-    if (a.n == 0) return hipSuccess;
+    if (nblocks == 0) return hipSuccess;
     if (split)
-        hipLaunchKernelGGL(k_force<true>, dim3(blocks_for(a.n), chunks), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_force<true>, dim3(nblocks), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL(k_force<false>, dim3(blocks_for(a.n), chunks), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_force<false>, dim3(nblocks), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s)
 {
-    if (a.n == 0) return hipSuccess;
+    if (a.ntot == 0) return hipSuccess;
     if (split)
-        hipLaunchKernelGGL(k_force_reduce<true>, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_force_reduce<true>, dim3(blocks_for(a.ntot)), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL(k_force_reduce<false>, dim3(blocks_for(a.n)), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_force_reduce<false>, dim3(blocks_for(a.ntot)), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -1,5 +1,12 @@
-// Launch interface between the C-ABI host code (capi.hip) and the gfx950
+// Launch interface between the C-ABI host code (capi_*.hip) and the gfx950
 // kernels.  Every launcher enqueues on `stream` and returns the launch status.
+//
+// All kernels are window-aware: a launch covers W independent systems
+// ("windows", e.g. umbrella-sampling windows of uneven size) whose atoms are
+// concatenated.  Window w owns atoms [a0, a0 + n) and MC points
+// [a0 nMC, (a0 + n) nMC).  A single system is the case W = 1.  Grids are 1-D
+// over a per-block work table (int4 {window, block, split/chunk, -}), so
+// uneven windows share one launch without idle blocks.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,9 +14,23 @@
 
 namespace isspa {
 
+struct WinDesc {
+    int a0, n;          // atoms of the window
+    int splits, aps;    // field pass: atom splits, atoms per split
+    int chunks, ppc;    // force pass: point chunks, points per chunk
+    int nbc, apc;       // nonbond: partner chunks, atoms per chunk
+    long long fpo;      // field partial offset:  sum over earlier windows of splits * P
+    long long ppo;      // force partial offset:  sum of chunks * n
+    long long npo;      // nonbond partial offset: sum of nbc * n
+    uint32_t sys;       // Philox counter word 3
+    float L, h;         // box length and half length
+    int pad;
+};
+
 // Per-type tables on the device, each type padded with one trailing zero row
 // so that a lerp at the last row reads 0 (the reference reads the next type's
-// first row there, 0 in every shipped table; quirk Q4).
+// first row there, 0 in every shipped table; quirk Q4).  All windows share the
+// grid; their types are concatenated.
 struct TableView {
     const float *g, *e, *f;
     int n_g, n_e, n_f;     // logical rows
@@ -29,25 +50,32 @@ struct TableView {
 constexpr int kRecFloats = 16;
 
 struct FieldArgs {
+    const WinDesc *wins;
+    const int4 *blocks;   // {w, point block, split, -}
     const float4 *pts;
     const float4 *xyzq;
     const int *itype;
     TableView tab;
-    int n, P, atoms_per_split;
-    float L, h, e0;
-    const int *pbc_flag;
-    float *part_g;      // [split][P]
-    int *part_cnt;      // [split][P]
-    double *part_s0;    // [split][3][P]  sum_out q r / d^3
-    double *part_ein;   // [split][3][P]  sum_in (-e0 q / d^3 + Etab / d) r
+    int nmc;
+    float e0;
+    const int *pbc_flag;  // per window
+    float *part_g;        // [F]
+    int *part_cnt;        // [F]
+    double *part_s0;      // [3][F]  sum_out q r / d^3
+    double *part_ein;     // [3][F]  sum_in (-e0 q / d^3 + Etab / d) r
+    long long F;          // total field partial length
 };
 
 struct EpilogueArgs {
+    const WinDesc *wins;
+    const int *atom_win;
     const float4 *pts;
-    int P, splits;
+    long long Ptot;
+    int nmc;
     const float *part_g;
     const int *part_cnt;
     const double *part_s0, *part_ein;
+    long long F;
     float e0, p0, q0, o0;
     float *rec;         // [P][16]
     double *g_out;      // [P]      (may be null)
@@ -57,53 +85,63 @@ struct EpilogueArgs {
 };
 
 struct ForceArgs {
-    const float4 *rec4;  // [P][4] float4 view of the records
+    const WinDesc *wins;
+    const int4 *blocks;   // {w, receiver block, chunk, -}
+    const float4 *rec4;   // [P][4] float4 view of the records
     const float4 *xyzq;
     const int *itype;
-    const float *rmax2;  // per type
+    const float *rmax2;   // per type
     TableView tab;
-    int n, P, pts_per_chunk;
-    float L, h;
+    int nmc;
     const int *pbc_flag;
-    double *part;        // [chunk][6][N] (S_e, S_lj) or [chunk][9][N] (pol, cdd, lj)
+    double *part;         // [comps][C] (S_e, S_lj) or (pol, lj, cdd)
+    long long C;
 };
 
 struct ReduceArgs {
+    const WinDesc *wins;
+    const int *atom_win;
     const double *part;
-    int n, chunks;
+    long long C;
+    int ntot;
     const float4 *xyzq;
     const int *itype;
-    const float *vtot;   // per type
-    float4 *f;           // += total (may be null)
+    const float *vtot;    // per type
+    float4 *f;            // += total (may be null)
     double *f_lj, *f_pol, *f_cdd;  // component outputs (split mode)
 };
 
-hipError_t launch_mc_points(const float4 *xyzq, const int *itype, const float *rmax,
-                            const float *rmax2, int n, int nmc, uint64_t seed, uint32_t sys,
-                            uint32_t step, const uint32_t *step_d, float4 *pts, hipStream_t s);
-hipError_t launch_field(const FieldArgs &a, int splits, hipStream_t s);
+hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const float4 *xyzq,
+                            const int *itype, const float *rmax, const float *rmax2, int ntot,
+                            int nmc, uint64_t seed, uint32_t step, const uint32_t *step_d,
+                            float4 *pts, hipStream_t s);
+hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s);
-hipError_t launch_force(const ForceArgs &a, int chunks, bool split, hipStream_t s);
+hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s);
 hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s);
 
 // ---- nonbond ----
 struct NonbondArgs {
+    const WinDesc *wins;
+    const int4 *blocks;    // {w, receiver block, partner chunk, -}
     const float4 *xyzq;
     const float4 *aux;     // per atom j: {4 R_j^2, cA_j, cB_j, LJ type (int bits)}
     const float2 *lj;      // [T*T] {12A, 6B}
-    int T, n, atoms_per_chunk;
+    int T;
     const uint32_t *excl;  // [N][4] bit window j - i + 64 in [0,128)
     const int *excl_ptr;   // CSR fallback (null when the windows are complete)
     const int *excl_idx;
-    float L, h, cfar;
+    float cfar;
     const int *pbc_flag;
-    double *part;          // [chunk][3][N] or [chunk][9][N] (lj, coul, pair)
+    double *part;          // [comps][B]
+    long long B;
 };
-hipError_t launch_nonbond(const NonbondArgs &a, int chunks, bool split, hipStream_t s);
-hipError_t launch_chunk_reduce3(const double *part, int n, int chunks, int comps, float4 *f,
-                                double *out0, double *out1, double *out2, hipStream_t s);
+hipError_t launch_nonbond(const NonbondArgs &a, int nblocks, bool split, hipStream_t s);
+hipError_t launch_chunk_reduce3(const WinDesc *wins, const int *atom_win, const double *part,
+                                long long B, int ntot, int comps, float4 *f, double *o0,
+                                double *o1, double *o2, hipStream_t s);
 
-// ---- bonded ----
+// ---- bonded (window-agnostic: terms carry global atom indices) ----
 struct BondedArgs {
     const float4 *xyzq;
     float L, h;
@@ -120,7 +158,7 @@ struct BondedArgs {
     const float2 *lj;      // [T*T] {A, B}
     int T;
     float4 *contrib;       // [2 nb + 3 na + 6 nd]
-    double *energy;        // [nb + na + nd] x {E} : per-term energies (torsion, 1-4 c, 1-4 lj for dih)
+    double *energy;        // per-term energies (3 per dihedral: torsion, 1-4 Coulomb, 1-4 LJ)
     int terms;
 };
 hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s);
@@ -130,14 +168,17 @@ hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, dou
                                 hipStream_t s);
 
 // ---- integrator / restraint / utilities ----
-hipError_t launch_leapfrog(int n, float4 *xyzq, float4 *vel, const float4 *f, float kT, float dt,
-                           uint32_t thr, float L, uint64_t seed, uint32_t sys, uint32_t step,
+// atom_win == null: every atom belongs to the single window `one`
+hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDesc &one,
+                           int ntot, float4 *xyzq, float4 *vel, const float4 *f, float kT,
+                           float dt, uint32_t thr, uint64_t seed, uint32_t step,
                            const uint32_t *step_d, hipStream_t s);
-hipError_t launch_us(int nbias, const int2 *atoms, const float *mass, float k0, float k1,
-                     float x0, float L, const float4 *xyzq, float4 *f, double *com,
-                     hipStream_t s);
-hipError_t launch_pbc_flag(const float4 *xyzq, int n, float rmax_all, float L, int *flag,
-                           hipStream_t s);
+// one workgroup per restraint; bias entries [boff[r], boff[r+1]) {atom, group}
+hipError_t launch_us(int nres, const int *boff, const int2 *atoms, const float *mass,
+                     const float2 *kx /* {k, x0} */, const float *L, const float4 *xyzq,
+                     float4 *f, double *com, hipStream_t s);
+hipError_t launch_pbc_flag(const WinDesc *wins, int W, const float4 *xyzq, float rmax_all,
+                           int *flag, hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_add4(float4 *dst, const float4 *src, int n, hipStream_t s);
 hipError_t launch_step_advance(uint32_t *step_d, hipStream_t s);

@@ -5,24 +5,26 @@
 // wave-uniform partner stream for the all-pairs term, per-term contribution
 // slots plus a per-atom gather for the bonded terms (no float atomics), and
 // single-workgroup FP64 tree reductions for the umbrella centres of mass.
+// The all-pairs, integrator and restraint kernels are window-aware (kernels.hpp).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace isspa {
 
 constexpr int kBlk = 256;
-static inline int nblocks(long n) { return (int)((n + kBlk - 1) / kBlk); }
+static inline int nblocks(long long n) { return (int)((n + kBlk - 1) / kBlk); }
 
 // ---------------------------------------------------------------------------
 // A.4 all-pairs nonbond (src/nonbond_force_cuda.cu:31-174):
 //   non-excluded pairs: (12A/d^12 - 6B/d^6)/d^2 + q_i q_j/d^3
 //   all pairs:          IS-SPA pair electrostatics fdir(d, R_j)
-// F_i += s r with r = min_image(x_i - x_j).
+// F_i += s r with r = min_image(x_i - x_j), j over the receiver's window.
 // ---------------------------------------------------------------------------
 template <bool PBC, bool SPLIT>
 __device__ __forceinline__ void nonbond_tile(const NonbondArgs &a, int i, int j0, int j1,
-                                             float4 xi, int ti, const uint32_t *ex, bool near,
-                                             const float2 *__restrict__ ljrow, double *acc)
+                                             float L, float h, float4 xi, const uint32_t *ex,
+                                             bool near, const float2 *__restrict__ ljrow,
+                                             double *acc)
 {
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;           // total (or LJ in split mode)
     float c0 = 0.f, c1 = 0.f, c2 = 0.f;           // Coulomb (split)
@@ -32,9 +34,9 @@ __device__ __forceinline__ void nonbond_tile(const NonbondArgs &a, int i, int j0
         const float4 aj = a.aux[j];
         float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
         if (PBC) {
-            rx = min_image1(rx, a.L, a.h);
-            ry = min_image1(ry, a.L, a.h);
-            rz = min_image1(rz, a.L, a.h);
+            rx = min_image1(rx, L, h);
+            ry = min_image1(ry, L, h);
+            rz = min_image1(rz, L, h);
         }
         const float d2 = dist2_f32(rx, ry, rz);
         const float inv = rsqrt_f32(d2);
@@ -92,19 +94,20 @@ __global__ __launch_bounds__(kBlk) void k_nonbond(NonbondArgs a)
     extern __shared__ float2 lds_lj[];
     for (int t = threadIdx.x; t < a.T * a.T; t += kBlk) lds_lj[t] = a.lj[t];
     __syncthreads();
-    const int i = blockIdx.x * kBlk + threadIdx.x;
-    const int chunk = blockIdx.y;
-    const int j0 = chunk * a.atoms_per_chunk;
-    const int j1 = min(a.n, j0 + a.atoms_per_chunk);
-    const bool valid = i < a.n;
-    const int ii = valid ? i : a.n - 1;  // idle lanes mirror the last atom, results dropped
-    const float4 xi = a.xyzq[ii];
-    const int ti = __float_as_int(a.aux[ii].w);
+    const int4 bt = a.blocks[blockIdx.x];
+    const WinDesc W = a.wins[bt.x];
+    const int il = bt.y * kBlk + threadIdx.x;
+    const int j0 = W.a0 + bt.z * W.apc;
+    const int j1 = min(W.a0 + W.n, j0 + W.apc);
+    const bool valid = il < W.n;
+    const int i = W.a0 + (valid ? il : W.n - 1);  // idle lanes mirror the last atom, results dropped
+    const float4 xi = a.xyzq[i];
+    const int ti = __float_as_int(a.aux[i].w);
     uint32_t ex[4];
-    for (int k = 0; k < 4; ++k) ex[k] = a.excl[4 * ii + k];
-    const bool pbc = *a.pbc_flag != 0;
-    // wave bounds for the near-diagonal test
-    const int wave_lo = (blockIdx.x * kBlk + (threadIdx.x & ~(kWave - 1)));
+    for (int k = 0; k < 4; ++k) ex[k] = a.excl[4 * i + k];
+    const bool pbc = a.pbc_flag[bt.x] != 0;
+    // first receiver of this wave, for the near-diagonal test
+    const int wave_lo = W.a0 + bt.y * kBlk + (threadIdx.x & ~(kWave - 1));
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const float2 *ljrow = lds_lj + ti * a.T;
     constexpr int kTile = 64;
@@ -112,40 +115,45 @@ __global__ __launch_bounds__(kBlk) void k_nonbond(NonbondArgs a)
         const int je = min(j1, jt + kTile);
         const bool near = (jt < wave_lo + kWave + 64) && (je > wave_lo - 64);
         if (pbc)
-            nonbond_tile<true, SPLIT>(a, ii, jt, je, xi, ti, ex, near, ljrow, acc);
+            nonbond_tile<true, SPLIT>(a, i, jt, je, W.L, W.h, xi, ex, near, ljrow, acc);
         else
-            nonbond_tile<false, SPLIT>(a, ii, jt, je, xi, ti, ex, near, ljrow, acc);
+            nonbond_tile<false, SPLIT>(a, i, jt, je, W.L, W.h, xi, ex, near, ljrow, acc);
     }
     if (!valid) return;
-    const size_t N = (size_t)a.n;
+    const long long o = W.npo + (long long)bt.z * W.n + il;
     const int comps = SPLIT ? 9 : 3;
-    double *o = a.part + (size_t)chunk * comps * N;
-    for (int k = 0; k < comps; ++k) o[k * N + i] = acc[k];
+    for (int k = 0; k < comps; ++k) a.part[k * a.B + o] = acc[k];
 }
 
-hipError_t launch_nonbond(const NonbondArgs &a, int chunks, bool split, hipStream_t s)
+hipError_t launch_nonbond(const NonbondArgs &a, int nb, bool split, hipStream_t s)
 {
-    if (a.n == 0) return hipSuccess;
+    if (nb == 0) return hipSuccess;
     const size_t lds = sizeof(float2) * a.T * a.T;
     if (split)
-        hipLaunchKernelGGL(k_nonbond<true>, dim3(nblocks(a.n), chunks), dim3(kBlk), lds, s, a);
+        hipLaunchKernelGGL(k_nonbond<true>, dim3(nb), dim3(kBlk), lds, s, a);
     else
-        hipLaunchKernelGGL(k_nonbond<false>, dim3(nblocks(a.n), chunks), dim3(kBlk), lds, s, a);
+        hipLaunchKernelGGL(k_nonbond<false>, dim3(nb), dim3(kBlk), lds, s, a);
     return hipGetLastError();
 }
 
-// sum [chunk][comps][N] partials over chunks in order; comps == 3: f += sum;
-// comps == 9: three FP64 outputs
-__global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const double *__restrict__ part, int n,
-                                                        int chunks, int comps, float4 *f,
-                                                        double *o0, double *o1, double *o2)
+// sum [comps][B] partials over the window's chunks in order; comps == 3:
+// f += sum; comps == 9: three FP64 outputs
+__global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restrict__ wins,
+                                                        const int *__restrict__ atom_win,
+                                                        const double *__restrict__ part,
+                                                        long long B, int ntot, int comps,
+                                                        float4 *f, double *o0, double *o1,
+                                                        double *o2)
 {
     const int i = blockIdx.x * kBlk + threadIdx.x;
-    if (i >= n) return;
-    const size_t N = (size_t)n;
+    if (i >= ntot) return;
+    const WinDesc W = wins[atom_win[i]];
+    const int il = i - W.a0;
     double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < chunks; ++c)
-        for (int k = 0; k < comps; ++k) s[k] += part[((size_t)c * comps + k) * N + i];
+    for (int c = 0; c < W.nbc; ++c) {
+        const long long o = W.npo + (long long)c * W.n + il;
+        for (int k = 0; k < comps; ++k) s[k] += part[k * B + o];
+    }
     if (comps == 3) {
         float4 v = f[i];
         v.x += (float)s[0];
@@ -161,12 +169,13 @@ __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const double *__restrict
     }
 }
 
-hipError_t launch_chunk_reduce3(const double *part, int n, int chunks, int comps, float4 *f,
-                                double *o0, double *o1, double *o2, hipStream_t s)
+hipError_t launch_chunk_reduce3(const WinDesc *wins, const int *atom_win, const double *part,
+                                long long B, int ntot, int comps, float4 *f, double *o0,
+                                double *o1, double *o2, hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_reduce3, dim3(nblocks(n)), dim3(kBlk), 0, s, part, n, chunks, comps,
-                       f, o0, o1, o2);
+    if (ntot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_chunk_reduce3, dim3(nblocks(ntot)), dim3(kBlk), 0, s, wins, atom_win,
+                       part, B, ntot, comps, f, o0, o1, o2);
     return hipGetLastError();
 }
 
@@ -401,23 +410,29 @@ hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, dou
 // ---------------------------------------------------------------------------
 // A.15 leap-frog + Andersen (src/leapfrog_cuda.cu:33-67).  On a collision the
 // velocity is redrawn from Maxwell-Boltzmann, then half-kicked (quirk Q21).
+// Atom i of window w draws with counter {i - a0_w, draw, step, sys_w}.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlk) void k_leapfrog(int n, float4 *xyzq, float4 *vel,
+__global__ __launch_bounds__(kBlk) void k_leapfrog(const WinDesc *__restrict__ wins,
+                                                   const int *__restrict__ atom_win, WinDesc one,
+                                                   int ntot,
+                                                   float4 *xyzq, float4 *vel,
                                                    const float4 *__restrict__ f, float kT,
-                                                   float dt, uint32_t thr, float L, uint64_t seed,
-                                                   uint32_t sys, uint32_t step_arg,
+                                                   float dt, uint32_t thr, uint64_t seed,
+                                                   uint32_t step_arg,
                                                    const uint32_t *__restrict__ step_d)
 {
     const int i = blockIdx.x * kBlk + threadIdx.x;
-    if (i >= n) return;
+    if (i >= ntot) return;
     const uint32_t step = step_d ? *step_d : step_arg;
-    const uint4 o = philox_draw(seed, (uint32_t)i, kStreamAndersen, 0u, step, sys);
+    const WinDesc W = atom_win ? wins[atom_win[i]] : one;
+    const uint32_t il = (uint32_t)(i - W.a0);
+    const uint4 o = philox_draw(seed, il, kStreamAndersen, 0u, step, W.sys);
     float4 v = vel[i];
     const float4 fi = f[i];
     float4 x = xyzq[i];
     const float m = v.w;
     if (o.x < thr) {
-        const uint4 z = philox_draw(seed, (uint32_t)i, kStreamAndersen, 1u, step, sys);
+        const uint4 z = philox_draw(seed, il, kStreamAndersen, 1u, step, W.sys);
         const float s24 = 5.9604644775390625e-08f;
         const float u1a = (float)((z.x >> 8) + 1u) * s24, u2a = (float)(z.y >> 8) * s24;
         const float u1b = (float)((z.z >> 8) + 1u) * s24, u2b = (float)(z.w >> 8) * s24;
@@ -438,6 +453,7 @@ __global__ __launch_bounds__(kBlk) void k_leapfrog(int n, float4 *xyzq, float4 *
     x.y += v.y * dt;
     x.z += v.z * dt;
     // wrap (src/cuda_vector_routines.h:49-68)
+    const float L = W.L;
     x.x = x.x > L ? x.x - L : (x.x < 0.0f ? x.x + L : x.x);
     x.y = x.y > L ? x.y - L : (x.y < 0.0f ? x.y + L : x.y);
     x.z = x.z > L ? x.z - L : (x.z < 0.0f ? x.z + L : x.z);
@@ -445,30 +461,36 @@ __global__ __launch_bounds__(kBlk) void k_leapfrog(int n, float4 *xyzq, float4 *
     xyzq[i] = x;
 }
 
-hipError_t launch_leapfrog(int n, float4 *xyzq, float4 *vel, const float4 *f, float kT, float dt,
-                           uint32_t thr, float L, uint64_t seed, uint32_t sys, uint32_t step,
+hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDesc &one,
+                           int ntot, float4 *xyzq, float4 *vel, const float4 *f, float kT,
+                           float dt, uint32_t thr, uint64_t seed, uint32_t step,
                            const uint32_t *step_d, hipStream_t s)
 {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_leapfrog, dim3(nblocks(n)), dim3(kBlk), 0, s, n, xyzq, vel, f, kT, dt,
-                       thr, L, seed, sys, step, step_d);
+    if (ntot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leapfrog, dim3(nblocks(ntot)), dim3(kBlk), 0, s, wins, atom_win, one, ntot,
+                       xyzq, vel, f, kT, dt, thr, seed, step, step_d);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
-// A.16 umbrella restraint (src/us_force_cuda.cu:14-47).  One workgroup: the
-// two centres of mass by an FP64 tree reduction (the reference synchronises
-// only within a block, quirk Q12), then each bias atom's force.  Atoms in the
-// list are unique, so the force updates do not race.
+// A.16 umbrella restraint (src/us_force_cuda.cu:14-47).  One workgroup per
+// restraint: the two centres of mass by an FP64 tree reduction (the
+// reference synchronises only within a block, quirk Q12), then each bias
+// atom's force.  Atoms in a restraint are unique, so the updates do not race.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlk) void k_us(int nbias, const int2 *__restrict__ atoms,
-                                             const float *__restrict__ mass, float k0, float k1,
-                                             float x0, float L, const float4 *__restrict__ xyzq,
-                                             float4 *f, double *com)
+__global__ __launch_bounds__(kBlk) void k_us(const int *__restrict__ boff,
+                                             const int2 *__restrict__ atoms,
+                                             const float *__restrict__ mass,
+                                             const float2 *__restrict__ kx,
+                                             const float *__restrict__ Ls,
+                                             const float4 *__restrict__ xyzq, float4 *f,
+                                             double *com)
 {
     __shared__ double red[6][kBlk];
+    const int r = blockIdx.x;
+    const int b0 = boff[r], b1 = boff[r + 1];
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = threadIdx.x; b < nbias; b += kBlk) {
+    for (int b = b0 + threadIdx.x; b < b1; b += kBlk) {
         const int2 at = atoms[b];
         const float4 x = xyzq[at.x];
         const double m = mass[b];
@@ -483,50 +505,54 @@ __global__ __launch_bounds__(kBlk) void k_us(int nbias, const int2 *__restrict__
             for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
         __syncthreads();
     }
-    const double h = 0.5 * (double)L;
+    const double L = Ls[r];
+    const double h = 0.5 * L;
     double dv[3];
     for (int k = 0; k < 3; ++k) {
-        double r = red[k][0] - red[3 + k][0];
-        if (r > h) r -= L;
-        else if (r < -h) r += L;
-        dv[k] = r;
+        double d = red[k][0] - red[3 + k][0];
+        if (d > h) d -= L;
+        else if (d < -h) d += L;
+        dv[k] = d;
     }
     const double d = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-    for (int b = threadIdx.x; b < nbias; b += kBlk) {
+    const float2 p = kx[r];  // {k, x0}: group 0 pulls with -k, group 1 with +k
+    for (int b = b0 + threadIdx.x; b < b1; b += kBlk) {
         const int2 at = atoms[b];
-        const double pre = (double)(at.y == 0 ? k0 : k1) * (d - (double)x0) * (double)mass[b] / d;
+        const double kk = at.y == 0 ? -(double)p.x : (double)p.x;
+        const double pre = kk * (d - (double)p.y) * (double)mass[b] / d;
         float4 v = f[at.x];
         v.x += (float)(pre * dv[0]);
         v.y += (float)(pre * dv[1]);
         v.z += (float)(pre * dv[2]);
         f[at.x] = v;
     }
-    if (com && threadIdx.x < 6) com[threadIdx.x] = red[threadIdx.x][0];
+    if (com && threadIdx.x < 6) com[6 * r + threadIdx.x] = red[threadIdx.x][0];
 }
 
-hipError_t launch_us(int nbias, const int2 *atoms, const float *mass, float k0, float k1,
-                     float x0, float L, const float4 *xyzq, float4 *f, double *com,
+hipError_t launch_us(int nres, const int *boff, const int2 *atoms, const float *mass,
+                     const float2 *kx, const float *L, const float4 *xyzq, float4 *f, double *com,
                      hipStream_t s)
 {
-    if (nbias == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_us, dim3(1), dim3(kBlk), 0, s, nbias, atoms, mass, k0, k1, x0, L, xyzq,
-                       f, com);
+    if (nres == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_us, dim3(nres), dim3(kBlk), 0, s, boff, atoms, mass, kx, L, xyzq, f, com);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // utilities
 // ---------------------------------------------------------------------------
-// flag = 1 when some pair separation may exceed half a box (minimum image
-// needed); 0 when (extent + rmax + margin) < L/2 on every axis, in which case
-// the single-image minimum image is the identity for every atom-atom and
-// point-atom pair and the kernels take their shift-free loops.
-__global__ __launch_bounds__(1024) void k_pbc_flag(const float4 *__restrict__ xyzq, int n,
-                                                   float rmax_all, float L, int *flag)
+// flag[w] = 1 when some pair separation of window w may exceed half a box
+// (minimum image needed); 0 when (extent + rmax + margin) < L/2 on every axis,
+// in which case the single-image minimum image is the identity for every
+// atom-atom and point-atom pair and the kernels take their shift-free loops.
+__global__ __launch_bounds__(256) void k_pbc_flag(const WinDesc *__restrict__ wins,
+                                                  const float4 *__restrict__ xyzq,
+                                                  float rmax_all, int *flag)
 {
-    __shared__ float lo[3][1024], hi[3][1024];
+    __shared__ float lo[3][256], hi[3][256];
+    const WinDesc W = wins[blockIdx.x];
     float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, u[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
-    for (int i = threadIdx.x; i < n; i += 1024) {
+    for (int i = W.a0 + threadIdx.x; i < W.a0 + W.n; i += 256) {
         const float4 x = xyzq[i];
         l[0] = fminf(l[0], x.x); u[0] = fmaxf(u[0], x.x);
         l[1] = fminf(l[1], x.y); u[1] = fmaxf(u[1], x.y);
@@ -537,7 +563,7 @@ __global__ __launch_bounds__(1024) void k_pbc_flag(const float4 *__restrict__ xy
         hi[k][threadIdx.x] = u[k];
     }
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
             for (int k = 0; k < 3; ++k) {
                 lo[k][threadIdx.x] = fminf(lo[k][threadIdx.x], lo[k][threadIdx.x + w]);
@@ -549,16 +575,17 @@ __global__ __launch_bounds__(1024) void k_pbc_flag(const float4 *__restrict__ xy
         int need = 0;
         for (int k = 0; k < 3; ++k) {
             const float ext = hi[k][0] - lo[k][0];
-            if (!(ext + rmax_all + 1.0e-2f < 0.5f * L)) need = 1;  // NaN -> need
+            if (!(ext + rmax_all + 1.0e-2f < W.h)) need = 1;  // NaN -> need
         }
-        *flag = need;
+        flag[blockIdx.x] = need;
     }
 }
 
-hipError_t launch_pbc_flag(const float4 *xyzq, int n, float rmax_all, float L, int *flag,
-                           hipStream_t s)
+hipError_t launch_pbc_flag(const WinDesc *wins, int W, const float4 *xyzq, float rmax_all,
+                           int *flag, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pbc_flag, dim3(1), dim3(1024), 0, s, xyzq, n, rmax_all, L, flag);
+    if (W == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pbc_flag, dim3(W), dim3(256), 0, s, wins, xyzq, rmax_all, flag);
     return hipGetLastError();
 }
 

@@ -32,6 +32,11 @@ ip = C.POINTER(C.c_int)
 fp = C.POINTER(C.c_float)
 
 
+class Windows(C.Structure):
+    """isspa_windows"""
+    _fields_ = [("n_windows", i32), ("window_atoms", vp), ("system_ids", vp), ("lbox", vp)]
+
+
 class ModelDesc(C.Structure):
     """isspa_model_desc"""
     _fields_ = [("n_atoms", i32), ("n_types", i32), ("n_mc", i32), ("n_g", i32), ("n_e", i32),
@@ -57,6 +62,8 @@ _SIGS = {
     "isspa_timer_end": ([vp, vp, fp], i32),
     "isspa_default_constants": ([fp, fp, fp, fp, fp], i32),
     "isspa_model_create": ([C.POINTER(ModelDesc), vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
+    "isspa_model_create_batch": ([C.POINTER(ModelDesc), C.POINTER(Windows), vp, vp, vp, vp, vp,
+                                  C.POINTER(vp)], i32),
     "isspa_model_destroy": ([vp], i32),
     "isspa_model_force": ([vp, vp, vp, u32, vp, vp], i32),
     "isspa_model_force_components": ([vp, vp, vp, vp, vp, u32, vp], i32),
@@ -64,8 +71,11 @@ _SIGS = {
     "isspa_model_profile": ([vp, vp, i32, fp, vp], i32),
     "isspa_model_set_tiling": ([vp, i32, i32], i32),
     "isspa_model_get_tiling": ([vp, ip, ip], i32),
+    "isspa_model_set_target_waves": ([vp, i32], i32),
     "nonbond_model_create": ([i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, f32, f32,
                               C.POINTER(vp)], i32),
+    "nonbond_model_create_batch": ([i32, i32, vp, vp, vp, vp, vp, vp, i32, vp, i32, vp, f32, f32,
+                                    C.POINTER(Windows), C.POINTER(vp)], i32),
     "nonbond_model_destroy": ([vp], i32),
     "nonbond_force": ([vp, vp, vp, vp], i32),
     "nonbond_force_components": ([vp, vp, vp, vp, vp, vp], i32),
@@ -75,6 +85,7 @@ _SIGS = {
     "bonded_force": ([vp, i32, vp, vp, vp, vp], i32),
     "leapfrog_andersen": ([i32, vp, vp, vp, f32, f32, f32, f32, u64, u32, u32, vp, vp], i32),
     "us_model_create": ([i32, i32, vp, vp, vp, f32, f32, C.POINTER(vp)], i32),
+    "us_model_create_batch": ([i32, i32, vp, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
     "us_model_destroy": ([vp], i32),
     "us_force": ([vp, vp, vp, vp, vp], i32),
     "md_engine_create": ([vp, vp, vp, vp, i32, vp, vp, vp, f32, f32, f32, f32, u64, u32,
@@ -85,6 +96,7 @@ _SIGS = {
     "md_engine_capture": ([vp, i32, vp], i32),
     "md_engine_replay": ([vp, i32, vp], i32),
     "md_engine_forces": ([vp, vp], i32),
+    "md_engine_get_com": ([vp, vp, vp], i32),
 }
 
 EXPORTED = tuple(_SIGS)

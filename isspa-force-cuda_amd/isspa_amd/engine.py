@@ -32,6 +32,13 @@ class GpuSystem:
         self.dev = torch.device(device)
         torch.cuda.set_device(self.dev)
         self.system_id = int(system_id)
+        self.windows = getattr(system, "windows", None)
+        self._win = None
+        if self.windows is not None:
+            # isspa_windows points into these arrays; keep them alive with the object
+            self._win_arrays = (_np(system.windows, np.int32), _np(system.window_sys, np.uint32),
+                                _np(system.window_lbox, np.float32))
+            self._win = capi.Windows(len(self.windows), *(ptr(a) for a in self._win_arrays))
         n = system.n_atoms
         self.x = torch.from_numpy(_np(system.xyzq, np.float32)).to(self.dev)
         self.v = torch.from_numpy(_np(system.vel, np.float32)).to(self.dev)
@@ -40,7 +47,8 @@ class GpuSystem:
         self.isspa = self._make_isspa() if isspa else None
         self.nonbond = self._make_nonbond() if nonbond else None
         self.bonded = self._make_bonded() if bonded and self._has_bonded() else None
-        self.us = self._make_us() if us and system.us is not None else None
+        self.restraints = self._restraints()
+        self.us = self._make_us() if us and self.restraints else None
         self.engine = None
 
     # ---- construction -------------------------------------------------------
@@ -62,10 +70,15 @@ class GpuSystem:
         d.seed = int(s.seed) & (2 ** 64 - 1)
         d.system_id = self.system_id
         h = C.c_void_p()
-        check(capi.lib.isspa_model_create(C.byref(d), ptr(_np(it.isspa_type, np.int32)),
-                                          ptr(_np(it.rmax, np.float32)), ptr(_np(it.g, np.float32)),
-                                          ptr(_np(it.e, np.float32)), ptr(_np(it.f, np.float32)),
-                                          C.byref(h)), "isspa_model_create")
+        arrs = [_np(it.isspa_type, np.int32), _np(it.rmax, np.float32), _np(it.g, np.float32),
+                _np(it.e, np.float32), _np(it.f, np.float32)]
+        if self._win is None:
+            check(capi.lib.isspa_model_create(C.byref(d), *(ptr(a) for a in arrs), C.byref(h)),
+                  "isspa_model_create")
+        else:
+            check(capi.lib.isspa_model_create_batch(C.byref(d), C.byref(self._win),
+                                                    *(ptr(a) for a in arrs), C.byref(h)),
+                  "isspa_model_create_batch")
         self.desc = d
         return h
 
@@ -76,10 +89,14 @@ class GpuSystem:
         args = [_np(s.ityp, np.int32), _np(s.nbparm, np.int32), _np(s.lj_a, np.float32),
                 _np(s.lj_b, np.float32), _np(s.nexcl_prefix, np.int32), _np(s.excl_list, np.int32),
                 _np(s.isspa.isspa_type, np.int32), _np(s.isspa.rmax, np.float32)]
-        check(capi.lib.nonbond_model_create(
-            s.n_atoms, s.n_lj_types, ptr(args[0]), ptr(args[1]), ptr(args[2]), ptr(args[3]),
-            ptr(args[4]), ptr(args[5]), int(len(s.excl_list)), ptr(args[6]), s.isspa.n_types,
-            ptr(args[7]), c["ep"], s.lbox, C.byref(h)), "nonbond_model_create")
+        common = (s.n_atoms, s.n_lj_types, ptr(args[0]), ptr(args[1]), ptr(args[2]), ptr(args[3]),
+                  ptr(args[4]), ptr(args[5]), int(len(s.excl_list)), ptr(args[6]),
+                  s.isspa.n_types, ptr(args[7]), c["ep"], s.lbox)
+        if self._win is None:
+            check(capi.lib.nonbond_model_create(*common, C.byref(h)), "nonbond_model_create")
+        else:
+            check(capi.lib.nonbond_model_create_batch(*common, C.byref(self._win), C.byref(h)),
+                  "nonbond_model_create_batch")
         return h
 
     def _has_bonded(self):
@@ -101,15 +118,35 @@ class GpuSystem:
             C.byref(h)), "bonded_model_create")
         return h
 
+    def _restraints(self):
+        """[(restraint, box)] in global atom indices: the system's one
+        restraint, or one per window of a batch (windows without one skipped)."""
+        s = self.system
+        if self.windows is None:
+            return [(s.us, s.lbox)] if s.us is not None else []
+        return [(r, float(L)) for r, L in zip(s.restraints, s.window_lbox) if r is not None]
+
     def _make_us(self):
         s = self.system
-        us = s.us
         h = C.c_void_p()
-        al = _np(us.atom_list, np.int32)
-        ms = _np(us.mass, np.float32)
-        kumb = us.kumb
-        check(capi.lib.us_model_create(s.n_atoms, len(al), ptr(al), ptr(ms), ptr(kumb), us.x0,
-                                       s.lbox, C.byref(h)), "us_model_create")
+        if self.windows is None:
+            us = s.us
+            al = _np(us.atom_list, np.int32)
+            ms = _np(us.mass, np.float32)
+            kumb = us.kumb
+            check(capi.lib.us_model_create(s.n_atoms, len(al), ptr(al), ptr(ms), ptr(kumb), us.x0,
+                                           s.lbox, C.byref(h)), "us_model_create")
+            return h
+        rs = self.restraints
+        boff = _np(np.concatenate([[0], np.cumsum([len(r.atom_list) for r, _ in rs])]), np.int32)
+        al = _np(np.concatenate([r.atom_list for r, _ in rs]), np.int32)
+        ms = _np(np.concatenate([r.mass for r, _ in rs]), np.float32)
+        k = _np([r.k for r, _ in rs], np.float32)
+        x0 = _np([r.x0 for r, _ in rs], np.float32)
+        L = _np([b for _, b in rs], np.float32)
+        check(capi.lib.us_model_create_batch(s.n_atoms, len(rs), ptr(boff), ptr(al), ptr(ms),
+                                             ptr(k), ptr(x0), ptr(L), C.byref(h)),
+              "us_model_create_batch")
         return h
 
     # ---- single terms (reference wrapper semantics: forces ADD into f) -----
@@ -151,6 +188,9 @@ class GpuSystem:
     def set_tiling(self, field_splits: int, force_chunks: int):
         check(capi.lib.isspa_model_set_tiling(self.isspa, field_splits, force_chunks))
 
+    def set_target_waves(self, waves: int):
+        check(capi.lib.isspa_model_set_target_waves(self.isspa, int(waves)))
+
     def tiling(self):
         a, b = C.c_int(), C.c_int()
         check(capi.lib.isspa_model_get_tiling(self.isspa, C.byref(a), C.byref(b)))
@@ -179,17 +219,27 @@ class GpuSystem:
 
     def us_force(self, f: Optional[torch.Tensor] = None):
         f = self.f if f is None else f
-        com = torch.zeros(6, dtype=torch.float64, device=self.dev)
+        com = torch.zeros(6 * len(self.restraints), dtype=torch.float64, device=self.dev)
         check(capi.lib.us_force(self.us, ptr(self.x), ptr(f), ptr(com), self.stream()), "us_force")
         return f, com
 
     def leapfrog(self, step: int = 0, f: Optional[torch.Tensor] = None):
+        """Standalone integrator call: one leapfrog_andersen per window (the
+        whole system when not batched), each with its own box and system id."""
         s = self.system
         f = self.f if f is None else f
-        check(capi.lib.leapfrog_andersen(s.n_atoms, ptr(self.x), ptr(self.v), ptr(f), s.kT, s.dt,
-                                         s.pnu, s.lbox, int(s.seed) & (2 ** 64 - 1),
-                                         self.system_id, step, None, self.stream()),
-              "leapfrog_andersen")
+        if self.windows is None:
+            parts = [(0, s.n_atoms, s.lbox, self.system_id)]
+        else:
+            a0 = np.concatenate([[0], np.cumsum(s.windows)[:-1]])
+            parts = list(zip(a0.tolist(), s.windows.tolist(), s.window_lbox.tolist(),
+                             s.window_sys.tolist()))
+        for a0, n, L, sid in parts:
+            off = 16 * int(a0)
+            check(capi.lib.leapfrog_andersen(int(n), ptr(self.x) + off, ptr(self.v) + off,
+                                             ptr(f) + off, s.kT, s.dt, s.pnu, float(L),
+                                             int(s.seed) & (2 ** 64 - 1), int(sid), step, None,
+                                             self.stream()), "leapfrog_andersen")
 
     # ---- whole step --------------------------------------------------------
     def make_engine(self):
@@ -214,6 +264,12 @@ class GpuSystem:
     def forces(self):
         check(capi.lib.md_engine_forces(self.make_engine(), self.stream()), "md_engine_forces")
         return self.f
+
+    def com(self):
+        """restraint centres of mass of the last engine force evaluation"""
+        out = torch.zeros(6 * max(1, len(self.restraints)), dtype=torch.float64, device=self.dev)
+        check(capi.lib.md_engine_get_com(self.make_engine(), ptr(out), self.stream()))
+        return out
 
     def _side(self):
         # graph capture needs a stream other than the null stream; default priority
