@@ -14,6 +14,15 @@ per-point and per-atom constants hoisted), i.e. 70 N P; the average launch
 time comes from HIP events around back-to-back launches on the stream the
 kernel runs on (isspa_model_profile).  Peak: 157.3 TFLOP/s FP32 vector.
 
+Batched umbrella windows (reported beside the headline, "batched"): 64
+windows of uneven size (32 two-atom LJ pairs, 24 chains of 22 atoms, 8
+systems of 220 atoms; each with a k = 10 distance restraint, centres 3-15 A),
+nMC=20, run as ONE batched model whose whole step is one graph replay, against
+the plain per-window loop (64 single-window engines, one graph replay each per
+step, back to back on one stream).  Both report aggregate window-ns/day
+(sum over windows of ns/day at dt = 2 fs); the launch geometry of the batched
+model is chosen by isspa_amd.autotune.
+
 CPU baseline: the reference's own src/isspa_force.omp.c (built in place into
 oracle/_ref by oracle/Makefile), cpu_ispa_force on the same 19,998-atom
 geometry with nMC=20 and the workload's per-type parabola and LJ-solvent
@@ -78,6 +87,83 @@ def cpu_baseline(system, threads, budget_s=30.0):
     return times[len(times) // 2], kind, len(times)
 
 
+def _time_replays(launch, steps):
+    """wall seconds for `steps` calls of launch(), bracketed by synchronisations"""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        launch()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def batched_bench(steps, plain_rounds, warmup, tune_reps):
+    """Batched 64-window step vs the plain per-window loop on the same GPU."""
+    import numpy as np
+    import torch
+    from isspa_amd import autotune, batch, capi
+    from isspa_amd.engine import GpuSystem
+
+    windows = batch.umbrella_windows()
+    merged = batch.merge_windows(windows)
+    W = len(windows)
+    g = GpuSystem(merged, "cuda:0")
+    tr = autotune.tune(g, reps=tune_reps, waves=(1024, 2048, 4096, 8192, 16384, 32768), log=log)
+    log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per IS-SPA pass")
+    g.set_step(0)
+    g.capture(1)
+    st = torch.cuda.Stream()
+    eng = g.make_engine()
+
+    def run_batch():
+        capi.check(capi.lib.md_engine_replay(eng, 1, st.cuda_stream))
+
+    with torch.cuda.stream(st):
+        _time_replays(run_batch, warmup)
+        t_b = _time_replays(run_batch, steps)
+    if not np.all(np.isfinite(g.x.cpu().numpy())):
+        raise RuntimeError("batched run produced non-finite positions")
+    plain = []
+    for w, s in enumerate(windows):
+        p = GpuSystem(s, "cuda:0", system_id=w)
+        p.set_step(0)
+        p.capture(1)
+        plain.append((p, p.make_engine()))
+
+    def run_plain():
+        for _, e in plain:
+            capi.check(capi.lib.md_engine_replay(e, 1, st.cuda_stream))
+
+    with torch.cuda.stream(st):
+        _time_replays(run_plain, max(1, warmup // 10))
+        t_p = _time_replays(run_plain, plain_rounds)
+    ms_b = 1e3 * t_b / steps
+    ms_p = 1e3 * t_p / plain_rounds
+    nsd_b = W * 0.1728 * steps / t_b
+    nsd_p = W * 0.1728 * plain_rounds / t_p
+    log(f"batched: {ms_b:.4f} ms per step of all {W} windows ({nsd_b:.1f} window-ns/day); "
+        f"plain loop: {ms_p:.4f} ms ({nsd_p:.1f} window-ns/day); ratio {nsd_b / nsd_p:.2f}")
+    kinds = [w.name.split("_")[1] for w in windows]
+    out = {
+        "workload": f"{W} umbrella windows: {kinds.count('ljpair')} two-atom LJ pairs, "
+                    f"{kinds.count('chain')} x 22-atom chain, {kinds.count('big')} x 220-atom "
+                    f"(10 chains); k=10 distance restraint, centres 3-15 A; nMC={merged.n_mc}",
+        "windows": W, "atoms": merged.n_atoms, "steps": steps, "plain_rounds": plain_rounds,
+        "ms_per_step": round(ms_b, 5),
+        "window_ns_per_day": round(nsd_b, 2),
+        "plain_loop_ms_per_step": round(ms_p, 5),
+        "plain_loop_window_ns_per_day": round(nsd_p, 2),
+        "speedup_vs_plain_loop": round(nsd_b / nsd_p, 3),
+        "autotune": {"kind": tr.kind, "value": list(tr.value), "pass_ms": round(tr.ms, 5),
+                     "candidates": len(tr.table)},
+    }
+    for p, _ in plain:
+        p.close()
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +173,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--plain-rounds", type=int, default=100)
+    ap.add_argument("--tiling", default="", help="explicit field splits,force chunks (no tuning)")
+    ap.add_argument("--target-waves", type=int, default=0, help="automatic tiling at this target")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run an untimed copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -96,7 +190,7 @@ def main():
 
     import numpy as np
     import torch
-    from isspa_amd import synth
+    from isspa_amd import autotune, synth
     from isspa_amd.engine import GpuSystem
 
     t0 = time.perf_counter()
@@ -104,7 +198,25 @@ def main():
     log(f"workload config[{args.config}]: N={s.n_atoms} nMC={s.n_mc} P={s.n_points} "
         f"box={s.lbox} built in {time.perf_counter() - t0:.1f}s")
     g = GpuSystem(s, "cuda:0")
+    tune = None
+    if args.tiling:
+        g.set_tiling(*(int(v) for v in args.tiling.split(",")))
+    elif args.target_waves:
+        g.set_target_waves(args.target_waves)
+    elif not args.no_tune:
+        tune = autotune.tune(g, reps=2, log=log)
+        log(f"autotune: {tune.kind} {tune.value} -> {tune.ms:.4f} ms per IS-SPA pass")
     splits, chunks = g.tiling()
+    if args.copy_mib:
+        # untimed plain copy of known size: the calibration kernel for the
+        # FETCH_SIZE / WRITE_SIZE counters
+        nbytes = args.copy_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
+        del src, dst
     g.set_step(0)
     g.capture(1)
     g.replay(args.warmup)
@@ -171,6 +283,13 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in prof.items()},
         "field_pass_tflops": round(field_tflops, 3),
     }
+    if tune is not None:
+        out["config"]["autotune"] = {"kind": tune.kind, "value": list(tune.value),
+                                     "pass_ms": round(tune.ms, 4), "candidates": len(tune.table)}
+    g.close()
+    if not args.no_batch:
+        out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,
+                                       tune_reps=3)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         res = cpu_baseline(s, threads)
@@ -185,7 +304,6 @@ def main():
                 "seconds_per_step": round(t_cpu, 3),
             }
             out["gpu_over_cpu_step_ratio"] = round(t_cpu / (ms_step * 1e-3), 1)
-    g.close()
     print(json.dumps(out), flush=True)
 
 

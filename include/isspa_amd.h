@@ -142,8 +142,9 @@ int isspa_model_get_field(isspa_model *m, float *pts4_d, double *g_d, int *cnt_d
 int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int reps, float *ms, void *stream);
 /* Tuning knobs (0 = automatic): atom splits of the field pass, point chunks of
  * the force pass, applied to every window.  get_tiling reports the largest
- * window's.  The automatic choice aims at `waves` wavefronts per pass
- * (default 8192) shared among windows in proportion to their work. */
+ * window's.  The automatic choice aims at no more than `waves` wavefronts
+ * per pass (default 32768, four rounds of the 8192 resident-wave slots),
+ * shared among windows in proportion to their work. */
 int isspa_model_set_tiling(isspa_model *m, int field_splits, int force_chunks);
 int isspa_model_get_tiling(const isspa_model *m, int *field_splits, int *force_chunks);
 int isspa_model_set_target_waves(isspa_model *m, int waves);
@@ -257,7 +258,8 @@ int md_engine_set_step(md_engine *e, uint32_t step, void *stream);
 int md_engine_step(md_engine *e, void *stream);
 /* capture `steps_per_graph` consecutive steps into a graph (done once) */
 int md_engine_capture(md_engine *e, int steps_per_graph, void *stream);
-/* replay the captured graph `replays` times */
+/* replay the captured graph `replays` times; ISSPA_ERR_STATE if the IS-SPA
+ * model's tiling was changed after the capture (capture again) */
 int md_engine_replay(md_engine *e, int replays, void *stream);
 /* forces only (zero, bonded, IS-SPA, nonbond, US) for the current step */
 int md_engine_forces(md_engine *e, void *stream);

@@ -75,15 +75,20 @@ static WinDesc base_desc(const Windows &L, int w)
     return d;
 }
 
-// enough waves to cover the 256 CUs several times over
-static constexpr long kDefaultTargetWaves = 8192;
+// 256 CUs x 4 SIMDs x 8 resident waves = 8192 wave slots; the IS-SPA kernels
+// use few enough VGPRs for full residency.  Work items are equal, so a grid
+// runs in whole rounds of 8192 waves: targets are multiples of a round and
+// the split never overshoots (a grid of 4.05 rounds takes 5).
+static constexpr long kWaveSlots = 8192;
+static constexpr long kDefaultTargetWaves = 4 * kWaveSlots;
 
-// pieces for `lanes` lanes doing `work` items each, aiming at `target` waves,
-// with at least `min_work` items per piece
+// pieces for `lanes` lanes doing `work` items each, at most `target` waves in
+// total, with at least `min_work` items per piece
 static int auto_pieces(long long lanes, long long work, long long min_work, double target)
 {
-    const long long waves = std::max(1LL, (lanes + 63) / 64);
-    long long s = (long long)std::ceil(target / (double)waves);
+    const long long lane_blocks = std::max(1LL, (lanes + 255) / 256);
+    const long long waves = 4 * lane_blocks;
+    long long s = (long long)std::floor(target / (double)waves);
     s = std::min(s, std::max(1LL, work / min_work));
     return (int)std::max(1LL, s);
 }
@@ -118,6 +123,9 @@ struct isspa_model {
     long long F = 0, C = 0, cap_field = 0, cap_force = 0;
     float4 *prof_f = nullptr;  // force sink of isspa_model_profile
     TableView tv{};
+    // bumped whenever the launch tables are rebuilt; a graph captured under
+    // another generation holds freed pointers and must not be replayed
+    unsigned generation = 0;
 };
 
 static void isspa_free(isspa_model *m)
@@ -180,6 +188,7 @@ static int isspa_tile(isspa_model *m)
     m->npblk = (int)pbv.size();
     m->F = F;
     m->C = C;
+    m->generation++;
     if (F > m->cap_field) {
         dfree(m->part_g); dfree(m->part_cnt); dfree(m->part_s0); dfree(m->part_ein);
         CHK(dalloc(&m->part_g, F));
@@ -590,7 +599,7 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
     for (int w = 0; w < Ly.W; ++w) {
         WinDesc d = base_desc(Ly, w);
         const long long nw = Ly.n[w];
-        const double tgt = std::max(1.0, kDefaultTargetWaves * ((double)nw * nw / work_tot));
+        const double tgt = std::max(1.0, kWaveSlots * ((double)nw * nw / work_tot));
         const int c = auto_pieces(nw, nw, 64, tgt);
         d.apc = (int)((nw + c - 1) / c);
         d.nbc = (int)((nw + d.apc - 1) / d.apc);
@@ -974,6 +983,7 @@ struct md_engine {
     double *com_d = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    unsigned captured_generation = 0;
 };
 
 static void engine_free(md_engine *e)
@@ -1111,6 +1121,7 @@ extern "C" int md_engine_capture(md_engine *e, int steps_per_graph, void *stream
     if (he != hipSuccess) return fail(ISSPA_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(he));
     e->graph = g;
     HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->captured_generation = e->isspa ? e->isspa->generation : 0;
     return ISSPA_OK;
 }
 
@@ -1118,6 +1129,8 @@ extern "C" int md_engine_replay(md_engine *e, int replays, void *stream)
 {
     if (!e || replays < 0) return fail(ISSPA_ERR_ARG, "md_engine_replay: bad argument");
     if (!e->exec) return fail(ISSPA_ERR_STATE, "md_engine_replay: no captured graph");
+    if (e->isspa && e->isspa->generation != e->captured_generation)
+        return fail(ISSPA_ERR_STATE, "md_engine_replay: the IS-SPA tiling changed after capture; capture again");
     for (int r = 0; r < replays; ++r) HIPCHK(hipGraphLaunch(e->exec, S(stream)));
     return ISSPA_OK;
 }

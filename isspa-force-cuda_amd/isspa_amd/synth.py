@@ -30,6 +30,7 @@ import math
 import numpy as np
 
 from .system import IsspaTables, System, UmbrellaRestraint
+from .tables import parabola_tables
 
 SPEC = {
     "chain_seed": 2024,
@@ -78,34 +79,17 @@ def _neighbours(n, bonds):
 
 
 def _tables(sig, eps, rng):
+    """Draw the per-type parabola and E-field parameters and build the tables
+    (isspa_amd.tables.parabola_tables)."""
     S = SPEC
     T = len(sig)
-    rows = S["table_rows"]
-    r = (S["table_r0"] + S["table_dr"] * np.arange(rows)).astype(np.float64)
     g0 = rng.uniform(*S["g0_range"], size=T)
     al = rng.uniform(*S["alpha_range"], size=T)
     ce = rng.uniform(-S["efield_amp"], S["efield_amp"], size=T)
-    G = np.zeros((T, rows), np.float32)
-    E = np.zeros((T, rows), np.float32)
-    F = np.zeros((T, rows), np.float32)
-    par = {}
-    for t in range(T):
-        x0 = sig[t] + 0.5
-        w = math.sqrt(g0[t] / al[t])
-        r_hi = x0 + math.sqrt((g0[t] - 1.0) / al[t])
-        s_ts = 0.5 * (sig[t] + S["solvent_sigma"])
-        e_ts = math.sqrt(eps[t] * S["solvent_eps"])
-        r_lo = max(x0 - w, S["core_frac"] * s_ts)
-        for b, rb in enumerate(r):
-            if rb < r_lo:
-                continue
-            G[t, b] = g0[t] - al[t] * (rb - x0) ** 2 if rb < r_hi else 1.0
-            sr6 = (s_ts / rb) ** 6
-            F[t, b] = 24.0 * e_ts / rb * (2.0 * sr6 * sr6 - sr6)
-            E[t, b] = ce[t] * math.exp(-0.5 * (rb - x0) ** 2)
-        par[t] = dict(x0=x0, g0=g0[t], alpha=al[t], w=w, r_lo=r_lo, r_hi=r_hi, sigma_ts=s_ts,
-                      eps_ts=e_ts, lj_A=4 * e_ts * s_ts ** 12, lj_B=4 * e_ts * s_ts ** 6)
-    return G, E, F, par
+    x0 = [sig[t] + 0.5 for t in range(T)]
+    return parabola_tables(sig, eps, x0, g0, al, ce, r0=S["table_r0"], dr=S["table_dr"],
+                           rows=S["table_rows"], core=S["core_frac"],
+                           solvent_sigma=S["solvent_sigma"], solvent_eps=S["solvent_eps"])
 
 
 def make_chain(seed: int = SPEC["chain_seed"]) -> System:
