@@ -286,6 +286,16 @@ def main():
     if tune is not None:
         out["config"]["autotune"] = {"kind": tune.kind, "value": list(tune.value),
                                      "pass_ms": round(tune.ms, 4), "candidates": len(tune.table)}
+    # HBM-side bytes per k_force launch from the committed FETCH_SIZE /
+    # WRITE_SIZE passes (copy-calibrated); they depend on the tiling, which is
+    # reported with them
+    tr_path = os.path.join(ROOT, "profiles", "k_force_config2_traffic.json")
+    if args.config == 2 and os.path.exists(tr_path):
+        with open(tr_path) as fh:
+            tr = json.load(fh)
+        out["roofline"]["traffic"] = tr["traffic_bytes"]
+        out["roofline"]["traffic_tiling"] = tr["tiling"]
+        out["roofline"]["traffic_source"] = tr["summary"]
     g.close()
     if not args.no_batch:
         out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,

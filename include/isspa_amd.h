@@ -208,6 +208,52 @@ int bonded_force(bonded_model *m, int terms, const float *xyzq4_d, float *f4_d,
                  double *energies_d, void *stream);
 
 /* ------------------------------------------------------------------------- */
+/* Differentiable IS-SPA + nonbond forces for force matching.  No reference   */
+/* counterpart: the reference's force path (src/isspa_force_cuda.cu:110-327,  */
+/* src/nonbond_force_cuda.cu:31-174) with its per-type tables replaced by the */
+/* analytic functions they are sampled from, differentiable in              */
+/* theta_t = (sigma, eps, x0, g0, alpha) per type and the charge per atom:    */
+/*   sigma_ts = (sigma + sig_s)/2, eps_ts = sqrt(eps eps_s)                    */
+/*   r_lo = max(x0 - sqrt(g0/alpha), core sigma_ts), r_hi = x0 + sqrt((g0-1)/alpha) */
+/*   S(d) = smoothstep((d - r_lo)/delta + 1)                                  */
+/*   g(d) = S(d) max(0, g0 - alpha (d - x0)^2) (d < r_hi), 1 beyond            */
+/*   E(d) = camp exp(-(d - x0)^2/2) S(d)                                      */
+/*   F(d) = 24 eps_ts/d (2 (sigma_ts/d)^12 - (sigma_ts/d)^6) S(d)              */
+/*   nonbond A_ij = 4 eps_ij sigma_ij^12, B_ij = 4 eps_ij sigma_ij^6 (Lorentz-Berthelot) */
+/* IS-SPA types double as LJ types.  MC points are drawn once per call at the */
+/* fixed `step` (common random numbers: they depend on positions and rmax    */
+/* only).  Computed in FP32 with FP64 accumulation.                           */
+/* ------------------------------------------------------------------------- */
+typedef struct isspa_diff isspa_diff;
+typedef struct {
+    int n_atoms, n_types, n_mc;
+    float lbox;
+    float e0, p0, q0, o0, ep;        /* as isspa_default_constants */
+    double sig_s, eps_s, core, delta; /* solvent sphere, core floor, switch width */
+    uint64_t seed;
+    uint32_t step;
+} isspa_diff_desc;
+/* isspa_type int[N] (0-based; also the LJ type), rmax float[T], camp double[T]
+ * (E-field amplitudes, fixed), exclusions as for nonbond_model_create.  At
+ * most 64 types.  `windows` may be NULL (one system). */
+int isspa_diff_create(const isspa_diff_desc *desc, const isspa_windows *windows,
+                      const int *isspa_type, const float *rmax, const double *camp,
+                      const int *nexcl_prefix, const int *excl_list, int excl_len,
+                      isspa_diff **out);
+int isspa_diff_destroy(isspa_diff *m);
+int isspa_diff_set_tiling(isspa_diff *m, int field_splits, int force_chunks);
+/* f_d double[N*3] = F_isspa + F_nonbond (overwritten) for positions xyz4_d
+ * (float4, .w ignored), charges q_d double[N], theta_d double[T*5] */
+int isspa_diff_forces(isspa_diff *m, const float *xyz4_d, const double *q_d,
+                      const double *theta_d, double *f_d, void *stream);
+/* adjoint for lam_d = dL/dF (double[N*3]): gq_d double[N] = dL/dq,
+ * gtheta_d double[T*5] = dL/dtheta (FP64 per-type sums in a fixed order),
+ * gatom_d double[N*5] per-atom contributions (may be NULL) */
+int isspa_diff_vjp(isspa_diff *m, const float *xyz4_d, const double *q_d,
+                   const double *theta_d, const double *lam_d, double *gq_d, double *gtheta_d,
+                   double *gatom_d, void *stream);
+
+/* ------------------------------------------------------------------------- */
 /* Leap-frog + Andersen: leapfrog_cuda (src/leapfrog_cuda.cu:73-86, kernel    */
 /* :33-67).  Philox keyed by (seed, step, atom) replaces the curand states     */
 /* (init_rand_states, src/init_rand.cu:10-30).  kT in kcal/mol, dt in AMBER    */

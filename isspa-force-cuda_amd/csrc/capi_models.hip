@@ -11,7 +11,7 @@
 #include <numeric>
 #include <vector>
 
-#include "capi_common.hpp"
+#include "models.hpp"
 
 using namespace isspa;
 
@@ -20,13 +20,6 @@ using namespace isspa;
 // ---------------------------------------------------------------------------
 // window layout
 // ---------------------------------------------------------------------------
-struct Windows {
-    int W = 0, ntot = 0;
-    std::vector<int> a0, n;
-    std::vector<uint32_t> sys;
-    std::vector<float> L;
-    std::vector<int> atom_win;  // host copy
-};
 
 static int build_windows(int ntot, const isspa_windows *win, uint32_t sys0, float L0, Windows &out)
 {
@@ -75,12 +68,6 @@ static WinDesc base_desc(const Windows &L, int w)
     return d;
 }
 
-// 256 CUs x 4 SIMDs x 8 resident waves = 8192 wave slots; the IS-SPA kernels
-// use few enough VGPRs for full residency.  Work items are equal, so a grid
-// runs in whole rounds of 8192 waves: targets are multiples of a round and
-// the split never overshoots (a grid of 4.05 rounds takes 5).
-static constexpr long kWaveSlots = 8192;
-static constexpr long kDefaultTargetWaves = 4 * kWaveSlots;
 
 // pieces for `lanes` lanes doing `work` items each, at most `target` waves in
 // total, with at least `min_work` items per piece
@@ -96,37 +83,6 @@ static int auto_pieces(long long lanes, long long work, long long min_work, doub
 // ---------------------------------------------------------------------------
 // IS-SPA model
 // ---------------------------------------------------------------------------
-struct isspa_model {
-    isspa_model_desc d{};
-    Windows lay;
-    long long P = 0;
-    float rmax_all = 0.f;
-    int *itype = nullptr, *atom_win = nullptr;
-    float *rmax = nullptr, *rmax2 = nullptr, *vtot = nullptr;
-    float *g = nullptr, *e = nullptr, *f = nullptr;
-    float4 *pts = nullptr;
-    float *rec = nullptr;
-    double *g_out = nullptr, *E_out = nullptr, *E0_out = nullptr;
-    int *cnt_out = nullptr;
-    int *pbc_flag = nullptr;
-    // tiling
-    int user_splits = 0, user_chunks = 0;
-    long target_waves = kDefaultTargetWaves;
-    std::vector<WinDesc> wins_h;
-    WinDesc *wins = nullptr;
-    int4 *fblk = nullptr, *pblk = nullptr;
-    int nfblk = 0, npblk = 0;
-    float *part_g = nullptr;
-    int *part_cnt = nullptr;
-    double *part_s0 = nullptr, *part_ein = nullptr;
-    double *part_f = nullptr;
-    long long F = 0, C = 0, cap_field = 0, cap_force = 0;
-    float4 *prof_f = nullptr;  // force sink of isspa_model_profile
-    TableView tv{};
-    // bumped whenever the launch tables are rebuilt; a graph captured under
-    // another generation holds freed pointers and must not be replayed
-    unsigned generation = 0;
-};
 
 static void isspa_free(isspa_model *m)
 {
@@ -168,8 +124,11 @@ static int isspa_tile(isspa_model *m)
         const int pblocks = (int)((P + 255) / 256), rblocks = (int)((n + 255) / 256);
         for (int b = 0; b < pblocks; ++b)
             for (int sp = 0; sp < d.splits; ++sp) fb.push_back({(long long)d.aps, make_int4(w, b, sp, 0)});
-        for (int b = 0; b < rblocks; ++b)
-            for (int ch = 0; ch < d.chunks; ++ch) pb.push_back({(long long)d.ppc, make_int4(w, b, ch, 0)});
+        // chunk-major: the receiver blocks of one point chunk are consecutive,
+        // so the round-robin dispatch over the 8 XCDs gives each XCD's L2 a
+        // share of them that reuses the chunk's records
+        for (int ch = 0; ch < d.chunks; ++ch)
+            for (int b = 0; b < rblocks; ++b) pb.push_back({(long long)d.ppc, make_int4(w, b, ch, 0)});
         m->wins_h[w] = d;
     }
     auto by_work = [](const std::pair<long long, int4> &a, const std::pair<long long, int4> &b) {
@@ -335,9 +294,9 @@ extern "C" int isspa_model_get_tiling(const isspa_model *m, int *field_splits, i
     return ISSPA_OK;
 }
 
-static FieldArgs field_args(isspa_model *m, const float4 *x)
+FieldArgs field_args(isspa_model *m, const float4 *x)
 {
-    FieldArgs fa;
+    FieldArgs fa{};
     fa.wins = m->wins; fa.blocks = m->fblk; fa.pts = m->pts; fa.xyzq = x; fa.itype = m->itype;
     fa.tab = m->tv; fa.nmc = m->d.n_mc; fa.e0 = m->d.e0; fa.pbc_flag = m->pbc_flag;
     fa.part_g = m->part_g; fa.part_cnt = m->part_cnt; fa.part_s0 = m->part_s0;
@@ -345,9 +304,9 @@ static FieldArgs field_args(isspa_model *m, const float4 *x)
     return fa;
 }
 
-static EpilogueArgs epilogue_args(isspa_model *m)
+EpilogueArgs epilogue_args(isspa_model *m)
 {
-    EpilogueArgs ea;
+    EpilogueArgs ea{};
     ea.wins = m->wins; ea.atom_win = m->atom_win; ea.pts = m->pts; ea.Ptot = m->P;
     ea.nmc = m->d.n_mc; ea.part_g = m->part_g; ea.part_cnt = m->part_cnt;
     ea.part_s0 = m->part_s0; ea.part_ein = m->part_ein; ea.F = m->F;
@@ -357,9 +316,9 @@ static EpilogueArgs epilogue_args(isspa_model *m)
     return ea;
 }
 
-static ForceArgs force_args(isspa_model *m, const float4 *x)
+ForceArgs force_args(isspa_model *m, const float4 *x)
 {
-    ForceArgs fa;
+    ForceArgs fa{};
     fa.wins = m->wins; fa.blocks = m->pblk; fa.rec4 = reinterpret_cast<const float4 *>(m->rec);
     fa.xyzq = x; fa.itype = m->itype; fa.rmax2 = m->rmax2; fa.tab = m->tv; fa.nmc = m->d.n_mc;
     fa.pbc_flag = m->pbc_flag; fa.part = m->part_f; fa.C = m->C;
@@ -369,7 +328,7 @@ static ForceArgs force_args(isspa_model *m, const float4 *x)
 static ReduceArgs reduce_args(isspa_model *m, const float4 *x, float4 *f, double *flj,
                               double *fpol, double *fcdd)
 {
-    ReduceArgs ra;
+    ReduceArgs ra{};
     ra.wins = m->wins; ra.atom_win = m->atom_win; ra.part = m->part_f; ra.C = m->C;
     ra.ntot = m->d.n_atoms; ra.xyzq = x; ra.itype = m->itype; ra.vtot = m->vtot; ra.f = f;
     ra.f_lj = flj; ra.f_pol = fpol; ra.f_cdd = fcdd;
@@ -476,22 +435,6 @@ extern "C" int isspa_model_get_field(isspa_model *m, float *pts4_d, double *g_d,
 // ---------------------------------------------------------------------------
 // nonbond model
 // ---------------------------------------------------------------------------
-struct nonbond_model {
-    Windows lay;
-    int n = 0, T = 0;
-    float cfar = 0.f;
-    float4 *aux = nullptr;
-    float2 *lj = nullptr;
-    uint32_t *excl = nullptr;
-    int *excl_ptr = nullptr, *excl_idx = nullptr;
-    int *pbc_flag = nullptr, *atom_win = nullptr;
-    std::vector<WinDesc> wins_h;
-    WinDesc *wins = nullptr;
-    int4 *blk = nullptr;
-    int nblk = 0;
-    long long B = 0;
-    double *part = nullptr;
-};
 
 static void nonbond_free(nonbond_model *m)
 {
@@ -657,12 +600,12 @@ extern "C" int nonbond_model_destroy(nonbond_model *m)
     return ISSPA_OK;
 }
 
-static int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
+int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
                        double *o1, double *o2, hipStream_t s)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
     HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
-    NonbondArgs a;
+    NonbondArgs a{};
     a.wins = m->wins; a.blocks = m->blk; a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T;
     a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
     a.pbc_flag = m->pbc_flag; a.part = m->part; a.B = m->B;
@@ -827,7 +770,7 @@ extern "C" int bonded_force(bonded_model *m, int terms, const float *xyzq4_d, fl
 {
     if (!m || !xyzq4_d || !f4_d) return fail(ISSPA_ERR_ARG, "bonded_force: null argument");
     hipStream_t s = S(stream);
-    BondedArgs a;
+    BondedArgs a{};
     a.xyzq = reinterpret_cast<const float4 *>(xyzq4_d);
     a.L = m->lbox; a.h = 0.5f * m->lbox;
     a.nb = m->nb; a.na = m->na; a.nd = m->nd;

@@ -19,7 +19,11 @@
 //    into FP64, and write per-chunk partials that a reduction kernel sums in a
 //    fixed order (quirk Q10).
 //  * every launch covers any number of independent windows (kernels.hpp).
+//  * the field and force kernels are templated on AN: table lookups (the
+//    plain path) or the analytic per-type functions of the differentiable
+//    model (diff_device.hpp); only the rare in-sphere branches differ.
 #include "device_common.hpp"
+#include "diff_device.hpp"
 #include "kernels.hpp"
 
 namespace isspa {
@@ -113,7 +117,21 @@ __device__ __forceinline__ void field_in_sphere(const TableView &tb, int tj, flo
     st.ez += (double)(c * rz);
 }
 
-template <bool PBC>
+// the same with the analytic g and E of the differentiable model
+__device__ __forceinline__ void field_in_sphere_an(const DiffType &T, float d2, float inv, float s,
+                                                   float rx, float ry, float rz, float e0,
+                                                   FieldRare &st)
+{
+    const float d = d2 * inv;
+    st.g *= diff_g(T, d);
+    st.cnt += 1;
+    const float c = __builtin_fmaf(-e0, s, diff_E(T, d) * inv);
+    st.ex += (double)(c * rx);
+    st.ey += (double)(c * ry);
+    st.ez += (double)(c * rz);
+}
+
+template <bool PBC, bool AN>
 __device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, float L, float h,
                                            float px, float py, float pz, float R2, float &ax,
                                            float &ay, float &az, FieldRare &st)
@@ -136,10 +154,16 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, f
         ax = __builtin_fmaf(so, rx, ax);
         ay = __builtin_fmaf(so, ry, ay);
         az = __builtin_fmaf(so, rz, az);
-        if (in) field_in_sphere(a.tab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0, st);
+        if (in) {
+            if (AN)
+                field_in_sphere_an(a.dtypes[a.itype[j]], d2, inv, s, rx, ry, rz, a.e0, st);
+            else
+                field_in_sphere(a.tab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0, st);
+        }
     }
 }
 
+template <bool AN>
 __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
 {
     const int4 bt = a.blocks[blockIdx.x];
@@ -159,9 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
         const int je = min(j1, jt + kFieldTile);
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         if (pbc)
-            field_tile<true>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
+            field_tile<true, AN>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
         else
-            field_tile<false>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
+            field_tile<false, AN>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
         sx += (double)ax;
         sy += (double)ay;
         sz += (double)az;
@@ -282,10 +306,11 @@ struct ForceAcc {
     double cx, cy, cz;  // CDD sum (split mode only)
 };
 
-template <bool PBC, bool SPLIT>
+template <bool PBC, bool SPLIT, bool AN>
 __device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, long long m1,
                                            float L, float h, float xi, float yi, float zi,
-                                           float R2, const float *__restrict__ Ftab, float &sx,
+                                           float R2, const float *__restrict__ Ftab, int ti,
+                                           float &sx,
                                            float &sy, float &sz, ForceAcc &acc)
 {
     const float4 *__restrict__ rec = a.rec4;
@@ -348,11 +373,15 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, lon
         }
         if (in) {
             const float d = d2 * rinv;
-            const int b = (int)((d - a.tab.f_r0) * a.tab.f_inv + 0.5f);
             float fs = 0.0f;
-            if (b >= 0 && b < a.tab.n_f) {
-                const float ph = (d - (a.tab.f_r0 + (float)b * a.tab.f_dr)) * a.tab.f_inv;
-                fs = __builtin_fmaf(Ftab[b + 1], ph, Ftab[b] * (1.0f - ph)) * w;
+            if (AN) {
+                fs = diff_F(a.dtypes[ti], d, rinv) * w;
+            } else {
+                const int b = (int)((d - a.tab.f_r0) * a.tab.f_inv + 0.5f);
+                if (b >= 0 && b < a.tab.n_f) {
+                    const float ph = (d - (a.tab.f_r0 + (float)b * a.tab.f_dr)) * a.tab.f_inv;
+                    fs = __builtin_fmaf(Ftab[b + 1], ph, Ftab[b] * (1.0f - ph)) * w;
+                }
             }
             const float fl = fs * rinv;
             acc.lx += (double)(fl * rx);
@@ -370,7 +399,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, lon
     }
 }
 
-template <bool SPLIT>
+template <bool SPLIT, bool AN>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 {
     const int4 bt = a.blocks[blockIdx.x];
@@ -385,16 +414,18 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const float4 xi = a.xyzq[i];
     const int ti = a.itype[i];
     const float R2 = valid ? a.rmax2[ti] : -1.0f;
-    const float *Ftab = a.tab.f + ti * a.tab.ld_f;
+    const float *Ftab = AN ? nullptr : a.tab.f + ti * a.tab.ld_f;
     const bool pbc = a.pbc_flag[bt.x] != 0;
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (long long mt = m0; mt < m1; mt += kForceTile) {
         const long long me = min(m1, mt + kForceTile);
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         if (pbc)
-            force_tile<true, SPLIT>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
+            force_tile<true, SPLIT, AN>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx, sy,
+                                        sz, acc);
         else
-            force_tile<false, SPLIT>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, sx, sy, sz, acc);
+            force_tile<false, SPLIT, AN>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx,
+                                         sy, sz, acc);
         acc.ex += (double)sx;
         acc.ey += (double)sy;
         acc.ez += (double)sz;
@@ -467,7 +498,10 @@ hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const floa
 hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_field, dim3(nblocks), dim3(kBlock), 0, s, a);
+    if (a.dtypes)
+        hipLaunchKernelGGL(k_field<true>, dim3(nblocks), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_field<false>, dim3(nblocks), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -481,10 +515,17 @@ hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s)
 hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    if (split)
-        hipLaunchKernelGGL(k_force<true>, dim3(nblocks), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_force<false>, dim3(nblocks), dim3(kBlock), 0, s, a);
+    if (a.dtypes) {
+        if (split)
+            hipLaunchKernelGGL((k_force<true, true>), dim3(nblocks), dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_force<false, true>), dim3(nblocks), dim3(kBlock), 0, s, a);
+    } else {
+        if (split)
+            hipLaunchKernelGGL((k_force<true, false>), dim3(nblocks), dim3(kBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_force<false, false>), dim3(nblocks), dim3(kBlock), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "diff_device.hpp"
+
 namespace isspa {
 
 struct WinDesc {
@@ -56,6 +58,7 @@ struct FieldArgs {
     const float4 *xyzq;
     const int *itype;
     TableView tab;
+    const DiffType *dtypes;  // non-null: analytic functions instead of tables
     int nmc;
     float e0;
     const int *pbc_flag;  // per window
@@ -92,6 +95,7 @@ struct ForceArgs {
     const int *itype;
     const float *rmax2;   // per type
     TableView tab;
+    const DiffType *dtypes;  // non-null: analytic F instead of the table
     int nmc;
     const int *pbc_flag;
     double *part;         // [comps][C] (S_e, S_lj) or (pol, lj, cdd)
@@ -166,6 +170,96 @@ hipError_t launch_bonded_gather(const float4 *contrib, const int *ptr, const int
                                 float4 *f, hipStream_t s);
 hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, double *out5,
                                 hipStream_t s);
+
+// ---- differentiable model: parameter preparation, forward reduction, adjoint ----
+struct DiffPrepArgs {
+    int T;
+    const double *theta;   // [T][5] sigma, eps, x0, g0, alpha
+    const double *camp;    // [T]
+    double sig_s, eps_s, core, delta;
+    DiffType *types;       // out [T]
+    float2 *nb_lj;         // out [T*T] {12A, 6B} for the nonbond kernel
+    float4 *nb_ld;         // out [T*T] {A, B, dA/dsigma_ij, dB/dsigma_ij}
+};
+struct DiffAtomArgs {
+    int n;
+    const float4 *xyz;     // positions (.w ignored)
+    const double *q;       // charges
+    const double *lam;     // [N][3] cotangent, or null (forward)
+    const int *itype;
+    const float *vtot, *rmax2;
+    float4 *xq;            // out: positions with FP32 charges
+    float4 *arec;          // out [N][4]: {x, R2}, {-q vtot lam, type}, {-vtot lam, q}, {lam, 0}
+};
+struct DiffReduceArgs {
+    int n;
+    const WinDesc *wins;
+    const int *atom_win;
+    const double *part;    // force partials [6][C] (S_e, S_lj)
+    long long C;
+    const int *itype;
+    const float *vtot;
+    const double *q;
+    double *se;            // out [N][3]
+    double *f;             // out [N][3] (overwritten)
+};
+struct VjpArgs {
+    const WinDesc *wins;
+    const int *atom_win;
+    const int4 *fblk, *pblk;   // point-lane and atom-lane block tables
+    int nmc;
+    long long Ptot;
+    const float4 *pts, *rec4, *arec, *xq;
+    const int *itype;
+    const float *rmax2;
+    const DiffType *dtypes;
+    const int *pbc_flag;
+    const double *g_out, *E_out;
+    const int *cnt_out;
+    float e0, p0, q0, o0;
+    double *ppart;             // [10][F] point-record bars
+    long long F;
+    float4 *prec;              // [P][4] {p, R2}, {dL/dE, g dL/dg}, {dL/dE0, w}, -
+    double *apart;             // [6][C] atom bars: q, theta(5)
+    long long C;
+};
+struct VjpNbArgs {
+    const WinDesc *wins;
+    const int4 *blocks;
+    const float4 *xq, *aux, *arec;
+    const uint32_t *excl;
+    const int *excl_ptr, *excl_idx;
+    float cfar;
+    const int *pbc_flag;
+    const float4 *ld;          // [T*T] {A, B, dA, dB}
+    int T;
+    double *part;              // [3][B] q, sigma, eps bars
+    long long B;
+};
+struct VjpFinishArgs {
+    int n;
+    const WinDesc *wins, *nb_wins;
+    const int *atom_win;
+    const double *apart;
+    long long C;
+    const double *nbpart;
+    long long B;
+    const int *itype;
+    const float *vtot;
+    const double *lam, *se, *theta;
+    double *gq;                // out [N]
+    double *gatom;             // out [N][5]
+};
+hipError_t launch_diff_prep_types(const DiffPrepArgs &a, hipStream_t s);
+hipError_t launch_diff_prep_atoms(const DiffAtomArgs &a, hipStream_t s);
+hipError_t launch_diff_reduce(const DiffReduceArgs &a, hipStream_t s);
+hipError_t launch_vjp_points(const VjpArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_vjp_epilogue(const VjpArgs &a, hipStream_t s);
+hipError_t launch_vjp_atoms(const VjpArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_vjp_nonbond(const VjpNbArgs &a, int nblocks, hipStream_t s);
+hipError_t launch_vjp_finish(const VjpFinishArgs &a, hipStream_t s);
+hipError_t launch_type_reduce(const double *gatom, const int *itype, int n, int T,
+                              double *gtheta, hipStream_t s);
 
 // ---- integrator / restraint / utilities ----
 // atom_win == null: every atom belongs to the single window `one`

@@ -137,7 +137,7 @@ hipError_t launch_nonbond(const NonbondArgs &a, int nb, bool split, hipStream_t 
 }
 
 // sum [comps][B] partials over the window's chunks in order; comps == 3:
-// f += sum; comps == 9: three FP64 outputs
+// f += sum (or, with f null, o0 += sum in FP64); comps == 9: three FP64 outputs
 __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restrict__ wins,
                                                         const int *__restrict__ atom_win,
                                                         const double *__restrict__ part,
@@ -154,12 +154,14 @@ __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restric
         const long long o = W.npo + (long long)c * W.n + il;
         for (int k = 0; k < comps; ++k) s[k] += part[k * B + o];
     }
-    if (comps == 3) {
+    if (comps == 3 && f) {
         float4 v = f[i];
         v.x += (float)s[0];
         v.y += (float)s[1];
         v.z += (float)s[2];
         f[i] = v;
+    } else if (comps == 3) {
+        for (int k = 0; k < 3; ++k) o0[3 * i + k] += s[k];
     } else {
         for (int k = 0; k < 3; ++k) {
             o0[3 * i + k] = s[k];

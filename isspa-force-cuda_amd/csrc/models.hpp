@@ -1,0 +1,81 @@
+// Device-side state of the C-ABI models, shared by the C-ABI translation
+// units (capi_models.hip, capi_diff.hip).  Not part of the public ABI.
+#pragma once
+
+#include <vector>
+
+#include "capi_common.hpp"
+
+using namespace isspa;
+
+// 256 CUs x 4 SIMDs x 8 resident waves = 8192 wave slots; the IS-SPA kernels
+// use few enough VGPRs for full residency.  Work items are equal, so a grid
+// runs in whole rounds of 8192 waves: targets are multiples of a round and
+// the split never overshoots (a grid of 4.05 rounds takes 5).
+static constexpr long kWaveSlots = 8192;
+static constexpr long kDefaultTargetWaves = 4 * kWaveSlots;
+
+struct Windows {
+    int W = 0, ntot = 0;
+    std::vector<int> a0, n;
+    std::vector<uint32_t> sys;
+    std::vector<float> L;
+    std::vector<int> atom_win;  // host copy
+};
+
+struct isspa_model {
+    isspa_model_desc d{};
+    Windows lay;
+    long long P = 0;
+    float rmax_all = 0.f;
+    int *itype = nullptr, *atom_win = nullptr;
+    float *rmax = nullptr, *rmax2 = nullptr, *vtot = nullptr;
+    float *g = nullptr, *e = nullptr, *f = nullptr;
+    float4 *pts = nullptr;
+    float *rec = nullptr;
+    double *g_out = nullptr, *E_out = nullptr, *E0_out = nullptr;
+    int *cnt_out = nullptr;
+    int *pbc_flag = nullptr;
+    // tiling
+    int user_splits = 0, user_chunks = 0;
+    long target_waves = kDefaultTargetWaves;
+    std::vector<WinDesc> wins_h;
+    WinDesc *wins = nullptr;
+    int4 *fblk = nullptr, *pblk = nullptr;
+    int nfblk = 0, npblk = 0;
+    float *part_g = nullptr;
+    int *part_cnt = nullptr;
+    double *part_s0 = nullptr, *part_ein = nullptr;
+    double *part_f = nullptr;
+    long long F = 0, C = 0, cap_field = 0, cap_force = 0;
+    float4 *prof_f = nullptr;  // force sink of isspa_model_profile
+    TableView tv{};
+    // bumped whenever the launch tables are rebuilt; a graph captured under
+    // another generation holds freed pointers and must not be replayed
+    unsigned generation = 0;
+};
+
+struct nonbond_model {
+    Windows lay;
+    int n = 0, T = 0;
+    float cfar = 0.f;
+    float4 *aux = nullptr;
+    float2 *lj = nullptr;
+    uint32_t *excl = nullptr;
+    int *excl_ptr = nullptr, *excl_idx = nullptr;
+    int *pbc_flag = nullptr, *atom_win = nullptr;
+    std::vector<WinDesc> wins_h;
+    WinDesc *wins = nullptr;
+    int4 *blk = nullptr;
+    int nblk = 0;
+    long long B = 0;
+    double *part = nullptr;
+};
+
+FieldArgs field_args(isspa_model *m, const float4 *x);
+EpilogueArgs epilogue_args(isspa_model *m);
+ForceArgs force_args(isspa_model *m, const float4 *x);
+// the all-pairs term for positions x: split -> three FP64 component outputs;
+// otherwise f (float4, +=) or, with f null, o0 (double[N*3], +=)
+int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
+                double *o1, double *o2, hipStream_t s);

@@ -45,6 +45,14 @@ class ModelDesc(C.Structure):
                 ("q0", f32), ("o0", f32), ("ep", f32), ("seed", u64), ("system_id", u32)]
 
 
+class DiffDesc(C.Structure):
+    """isspa_diff_desc"""
+    _fields_ = [("n_atoms", i32), ("n_types", i32), ("n_mc", i32), ("lbox", f32), ("e0", f32),
+                ("p0", f32), ("q0", f32), ("o0", f32), ("ep", f32), ("sig_s", C.c_double),
+                ("eps_s", C.c_double), ("core", C.c_double), ("delta", C.c_double),
+                ("seed", u64), ("step", u32)]
+
+
 _SIGS = {
     "isspa_last_error": ([], C.c_char_p),
     "isspa_abi_version": ([], i32),
@@ -97,6 +105,12 @@ _SIGS = {
     "md_engine_replay": ([vp, i32, vp], i32),
     "md_engine_forces": ([vp, vp], i32),
     "md_engine_get_com": ([vp, vp, vp], i32),
+    "isspa_diff_create": ([C.POINTER(DiffDesc), C.POINTER(Windows), vp, vp, vp, vp, vp, i32,
+                           C.POINTER(vp)], i32),
+    "isspa_diff_destroy": ([vp], i32),
+    "isspa_diff_set_tiling": ([vp, i32, i32], i32),
+    "isspa_diff_forces": ([vp, vp, vp, vp, vp, vp], i32),
+    "isspa_diff_vjp": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -81,7 +81,8 @@ def parabola_tables(sigma: Sequence[float], eps: Sequence[float], x0: Sequence[f
             F[t, b] = 24.0 * e_ts / rb * (2.0 * sr6 * sr6 - sr6)
             E[t, b] = amp[t] * math.exp(-0.5 * (rb - x0[t]) ** 2)
         par[t] = dict(x0=x0[t], g0=g0[t], alpha=alpha[t], w=math.sqrt(g0[t] / alpha[t]),
-                      r_lo=r_lo, r_hi=r_hi, sigma_ts=s_ts, eps_ts=e_ts,
+                      r_lo=r_lo, r_hi=r_hi, sigma_ts=s_ts, eps_ts=e_ts, c=float(amp[t]),
+                      sigma=float(sigma[t]), eps=float(eps[t]),
                       lj_A=4 * e_ts * s_ts ** 12, lj_B=4 * e_ts * s_ts ** 6)
     return G, E, F, par
 

@@ -124,6 +124,34 @@ void orc_cpu_ispa_force(float *xyz, float *f, float *w, float *x0, float *g0, fl
                         int nThreads);
 void orc_srand(unsigned int s);
 
+/* ---- differentiable model (oracle_diff.c) ----
+ * Per-type trainable parameters (sigma, eps, x0, g0, alpha) and fixed
+ * constants of the analytic IS-SPA + nonbond model. */
+typedef struct {
+    int n_types;
+    const double *sigma, *eps, *x0, *g0, *alpha; /* [T] trainable */
+    const double *camp;                          /* [T] E-field amplitude */
+    const float *rmax;                           /* [T] sampling radius (FP32, as the device) */
+    const double *vtot;                          /* [T] MC normalisation */
+    double sig_s, eps_s, core, delta;            /* solvent sphere, core floor, switch width */
+    double e0, p0, q0, o0, ep;
+} orc_dparams;
+
+/* val = {g, E, F}(d) of type t; dval[5k+p] = d val[k] / d theta_p */
+void orc_diff_type_fn(const orc_dparams *P, int t, double d, double val[3], double dval[15]);
+/* Langevin moments and their derivatives */
+void orc_langevin_derivs(double x, double m[3], double dm[3]);
+/* forces of the analytic model: f_isspa, f_nb double[N*3]; psi (optional)
+ * double[P*8] = E(3), E0(3), g, cnt per point.  q double[N] are the charges
+ * (the .w of xyz4 is ignored); pre/list the exclusions. */
+void orc_diff_forward(int n, int nmc, const float *xyz4, const double *q, const int *itype,
+                      const float *pts, float lbox, const orc_dparams *P, const int *pre,
+                      const int *list, double *f_isspa, double *f_nb, double *psi);
+/* adjoint: lam = dL/dF (F = f_isspa + f_nb) -> gq[N] = dL/dq, gt[T*5] = dL/dtheta */
+void orc_diff_vjp(int n, int nmc, const float *xyz4, const double *q, const int *itype,
+                  const float *pts, float lbox, const orc_dparams *P, const int *pre,
+                  const int *list, const double *lam, double *gq, double *gt);
+
 #ifdef __cplusplus
 }
 #endif

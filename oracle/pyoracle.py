@@ -277,3 +277,99 @@ def cpu_ispa_force(fn, xyz, w, x0, g0, gr2, alpha, lj_A, lj_B, ityp, n_mc, n_thr
        (w, x0, g0, gr2, alpha, lj_A, lj_B)), np.ascontiguousarray(ityp, np.int32), n, n_mc,
        n_threads)
     return f
+
+
+# ---- differentiable model (oracle_diff.c) --------------------------------
+class DParams(C.Structure):
+    _fields_ = [("n_types", C.c_int), ("sigma", C.c_void_p), ("eps", C.c_void_p),
+                ("x0", C.c_void_p), ("g0", C.c_void_p), ("alpha", C.c_void_p),
+                ("camp", C.c_void_p), ("rmax", C.c_void_p), ("vtot", C.c_void_p),
+                ("sig_s", C.c_double), ("eps_s", C.c_double), ("core", C.c_double),
+                ("delta", C.c_double), ("e0", C.c_double), ("p0", C.c_double),
+                ("q0", C.c_double), ("o0", C.c_double), ("ep", C.c_double)]
+
+
+class _DHold:
+    """keeps the arrays behind a DParams alive"""
+
+    def __init__(self, system, prm):
+        c = consts()
+        self.arrs = [np.ascontiguousarray(getattr(prm, k), np.float64)
+                     for k in ("sigma", "eps", "x0", "g0", "alpha", "camp")]
+        self.rmax = np.ascontiguousarray(system.isspa.rmax, np.float32)
+        self.vtot = system.isspa.vtot(system.n_mc).astype(np.float64)
+        p = DParams()
+        p.n_types = len(self.arrs[0])
+        p.sigma, p.eps, p.x0, p.g0, p.alpha, p.camp = (a.ctypes.data for a in self.arrs)
+        p.rmax, p.vtot = self.rmax.ctypes.data, self.vtot.ctypes.data
+        p.sig_s, p.eps_s, p.core, p.delta = prm.sig_s, prm.eps_s, prm.core, prm.delta
+        p.e0, p.p0, p.q0, p.o0, p.ep = c.e0, c.p0, c.q0, c.o0, c.ep
+        self.p = p
+
+
+def _diff_sigs(L):
+    if getattr(L, "_diff_ready", False):
+        return
+    L.orc_diff_type_fn.argtypes = [C.POINTER(DParams), C.c_int, C.c_double, f64p, f64p]
+    L.orc_langevin_derivs.argtypes = [C.c_double, f64p, f64p]
+    L.orc_diff_forward.argtypes = [C.c_int, C.c_int, f32p, f64p, i32p, f32p, C.c_float,
+                                   C.POINTER(DParams), i32p, i32p, f64p, f64p, f64p]
+    L.orc_diff_vjp.argtypes = [C.c_int, C.c_int, f32p, f64p, i32p, f32p, C.c_float,
+                               C.POINTER(DParams), i32p, i32p, f64p, f64p, f64p]
+    L._diff_ready = True
+
+
+def diff_type_fn(system, prm, t: int, d: float):
+    """(val[3] = g, E, F; dval[3, 5]) of the analytic model at distance d"""
+    L = lib()
+    _diff_sigs(L)
+    h = _DHold(system, prm)
+    v, dv = np.zeros(3), np.zeros(15)
+    L.orc_diff_type_fn(C.byref(h.p), t, d, v, dv)
+    return v, dv.reshape(3, 5)
+
+
+def langevin_derivs(x: float):
+    L = lib()
+    _diff_sigs(L)
+    m, dm = np.zeros(3), np.zeros(3)
+    L.orc_langevin_derivs(x, m, dm)
+    return m, dm
+
+
+def _diff_inputs(system, pts, xyzq):
+    xyzq = np.ascontiguousarray(system.xyzq if xyzq is None else xyzq, np.float32)
+    return (xyzq, np.ascontiguousarray(system.isspa.isspa_type, np.int32),
+            np.ascontiguousarray(pts, np.float32), np.ascontiguousarray(system.nexcl_prefix, np.int32),
+            np.ascontiguousarray(system.excl_list, np.int32))
+
+
+def diff_forward(system, prm, pts, xyzq=None):
+    """{'isspa', 'nb', 'total': double[N,3], 'psi': double[P,8]} of the
+    analytic model; charges are prm.q."""
+    L = lib()
+    _diff_sigs(L)
+    h = _DHold(system, prm)
+    x, it, p, pre, lst = _diff_inputs(system, pts, xyzq)
+    n = system.n_atoms
+    fi, fn = np.zeros((n, 3)), np.zeros((n, 3))
+    psi = np.zeros((n * system.n_mc, 8))
+    q = np.ascontiguousarray(prm.q, np.float64)
+    L.orc_diff_forward(n, system.n_mc, x, q, it, p, system.lbox, C.byref(h.p), pre, lst, fi, fn,
+                       psi)
+    return {"isspa": fi, "nb": fn, "total": fi + fn, "psi": psi}
+
+
+def diff_vjp(system, prm, pts, lam, xyzq=None):
+    """(dL/dq double[N], dL/dtheta double[T,5]) for cotangent lam = dL/dF."""
+    L = lib()
+    _diff_sigs(L)
+    h = _DHold(system, prm)
+    x, it, p, pre, lst = _diff_inputs(system, pts, xyzq)
+    n = system.n_atoms
+    gq = np.zeros(n)
+    gt = np.zeros((h.p.n_types, 5))
+    q = np.ascontiguousarray(prm.q, np.float64)
+    L.orc_diff_vjp(n, system.n_mc, x, q, it, p, system.lbox, C.byref(h.p), pre, lst,
+                   np.ascontiguousarray(lam, np.float64), gq, gt)
+    return gq, gt
