@@ -1,0 +1,162 @@
+"""Force matching with the differentiable IS-SPA + nonbond model.
+
+Workload: a batch of configurations of one topology (by default 256 frames of
+the 1,100-atom, 50-chain system, sampled from its own MD at 298 K), each frame
+a window of one DiffForces model with fixed common-random-number MC points.
+Target forces come from the model at the true parameters.  The fit starts
+from perturbed parameters (sigma +-5 %, eps +-10 %, charges +-0.05 e,
+g0 and alpha +-10 %) and runs Adam on
+
+    L = mean_i |F_i(theta, q) - F_i^target|^2 / mean_i |F_i^target|^2
+
+with gradients from the hand-written adjoint (isspa_forces).  Charges are
+per topology atom, shared by every frame.
+
+    python -m isspa_amd.forcematch [--frames 256] [--steps 200] [--lr 0.01]
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import sys
+import time
+from typing import List
+
+import numpy as np
+
+from .system import System
+
+AMBER_CHARGE = 18.2223  # e -> AMBER internal charge units
+
+
+def sample_frames(system: System, n_frames: int, stride: int = 20, warmup: int = 200,
+                  device: str = "cuda:0") -> List[np.ndarray]:
+    """Positions (float32[N, 3]) every `stride` MD steps after `warmup`
+    steps of the plain engine (bonded + IS-SPA + nonbond + Andersen)."""
+    import torch
+
+    from .engine import GpuSystem
+    g = GpuSystem(system, device)
+    g.set_step(0)
+    g.capture(1)
+    g.replay(warmup)
+    frames = []
+    for _ in range(n_frames):
+        g.replay(stride)
+        torch.cuda.synchronize()
+        frames.append(g.x[:, :3].cpu().numpy().copy())
+    g.close()
+    return frames
+
+
+def frame_batch(system: System, frames: List[np.ndarray]) -> System:
+    """One batched System whose windows are the frames (same topology)."""
+    from .batch import merge_windows
+    ws = []
+    for x in frames:
+        s = dataclasses.replace(system)
+        s.xyzq = system.xyzq.copy()
+        s.xyzq[:, :3] = x
+        ws.append(s)
+    return merge_windows(ws, name=f"{system.name}x{len(frames)}frames")
+
+
+def perturb(theta: np.ndarray, q: np.ndarray, rng, sigma=0.05, eps=0.10, dq_e=0.05, g0a=0.10):
+    th = theta.copy()
+    T = th.shape[0]
+    th[:, 0] *= 1.0 + rng.uniform(-sigma, sigma, T)
+    th[:, 1] *= 1.0 + rng.uniform(-eps, eps, T)
+    th[:, 3] *= 1.0 + rng.uniform(-g0a, g0a, T)
+    th[:, 4] *= 1.0 + rng.uniform(-g0a, g0a, T)
+    # keep g0 > 1 (the parabola must reach g = 1)
+    th[:, 3] = np.maximum(th[:, 3], 1.05)
+    return th, q + rng.uniform(-dq_e, dq_e, q.shape) * AMBER_CHARGE
+
+
+def fit(model, n_frames: int, theta0, q0, target, steps: int = 200, lr: float = 0.01,
+        log=None):
+    """Adam in relative units for theta and in 0.05 e units for the charges.
+    Returns (theta, q, history)."""
+    import torch
+
+    from .diff import isspa_forces
+    dev = target.device
+    th_base = torch.as_tensor(theta0, dtype=torch.float64, device=dev)
+    q_base = torch.as_tensor(q0, dtype=torch.float64, device=dev)
+    rel = torch.zeros_like(th_base, requires_grad=True)
+    dq = torch.zeros_like(q_base, requires_grad=True)
+    qs = 0.05 * AMBER_CHARGE
+    opt = torch.optim.Adam([rel, dq], lr=lr)
+    norm = float((target ** 2).mean())
+    hist = []
+    for it in range(steps):
+        opt.zero_grad()
+        th = th_base * (1.0 + rel)
+        q = (q_base + qs * dq).repeat(n_frames)
+        f = isspa_forces(th, q, model)
+        L = ((f - target) ** 2).mean() / norm
+        L.backward()
+        opt.step()
+        hist.append(L.item())
+        if log and (it % 20 == 0 or it == steps - 1):
+            log(f"step {it:4d}  loss {hist[-1]:.4e}")
+    with torch.no_grad():
+        return (th_base * (1.0 + rel)).cpu().numpy(), (q_base + qs * dq).cpu().numpy(), hist
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--frames", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--lr", type=float, default=0.01)
+    ap.add_argument("--stride", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=2025)
+    args = ap.parse_args(argv)
+    import torch
+
+    from . import synth
+    from .diff import DiffForces, DiffParams
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    base = synth.config(1)
+    prm = DiffParams.from_system(base)
+    t0 = time.perf_counter()
+    frames = sample_frames(base, args.frames, stride=args.stride)
+    sysb = frame_batch(base, frames)
+    log(f"{args.frames} frames of {base.n_atoms} atoms sampled in {time.perf_counter() - t0:.1f}s")
+    prm_b = dataclasses.replace(prm, q=np.tile(prm.q, args.frames))
+    model = DiffForces(sysb, prm_b, step=0)
+    dev = model.dev
+    th_true = prm.theta
+    q_true = prm.q
+    target = model.forces(torch.as_tensor(np.tile(q_true, args.frames), device=dev),
+                          torch.as_tensor(th_true, device=dev))
+    rng = np.random.default_rng(args.seed)
+    th0, q0 = perturb(th_true, q_true, rng)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    th, q, hist = fit(model, args.frames, th0, q0, target, steps=args.steps, lr=args.lr, log=log)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t1
+    rel_err = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
+    out = {
+        "workload": f"{args.frames} frames x {base.n_atoms} atoms, nMC={base.n_mc}",
+        "steps": args.steps, "lr": args.lr,
+        "loss_first": hist[0], "loss_last": hist[-1],
+        "seconds_per_step": wall / args.steps,
+        "theta_max_rel_err_start": {n: rel_err(th0[:, k], th_true[:, k])
+                                    for k, n in enumerate(("sigma", "eps", "x0", "g0", "alpha"))},
+        "theta_max_rel_err_end": {n: rel_err(th[:, k], th_true[:, k])
+                                  for k, n in enumerate(("sigma", "eps", "x0", "g0", "alpha"))},
+        "charge_max_abs_err_e_start": float(np.abs(q0 - q_true).max() / AMBER_CHARGE),
+        "charge_max_abs_err_e_end": float(np.abs(q - q_true).max() / AMBER_CHARGE),
+    }
+    print(json.dumps(out), flush=True)
+    model.close()
+
+
+if __name__ == "__main__":
+    main()

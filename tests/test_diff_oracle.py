@@ -142,3 +142,22 @@ def test_adjoint_matches_central_differences(small):
         qm[i] -= h
         fd = (L(prm.with_theta(th, qp)) - L(prm.with_theta(th, qm))) / (2 * h)
         assert abs(fd - gq[i]) <= 1e-5 * max(abs(fd), 1e-3 * scale_q), (i, fd, gq[i])
+
+
+def test_forcematch_perturbation_and_frame_batch(small):
+    """Perturbation ranges of the force-matching workload, and frames of one
+    topology batched as windows sharing one type block."""
+    from isspa_amd import forcematch
+    prm = DiffParams.from_system(small)
+    rng = np.random.default_rng(0)
+    th, q = forcematch.perturb(prm.theta, prm.q, rng)
+    r = th / prm.theta - 1.0
+    assert np.all(np.abs(r[:, 0]) <= 0.05) and np.all(np.abs(r[:, 1]) <= 0.10)
+    assert np.all(r[:, 2] == 0.0)
+    assert np.all(np.abs(r[:, 3:]) <= 0.10 + 1e-12) or np.all(th[:, 3] >= 1.05)
+    assert np.all(np.abs(q - prm.q) <= 0.05 * forcematch.AMBER_CHARGE)
+    frames = [small.xyzq[:, :3] + np.float32(0.01 * k) for k in range(3)]
+    b = forcematch.frame_batch(small, frames)
+    assert list(b.windows) == [small.n_atoms] * 3
+    assert b.isspa.n_types == prm.n_types and b.n_lj_types == small.n_lj_types
+    assert np.array_equal(b.xyzq[small.n_atoms:2 * small.n_atoms, :3], frames[1])
