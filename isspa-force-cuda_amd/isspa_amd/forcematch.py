@@ -12,7 +12,7 @@ g0 and alpha +-10 %) and runs Adam on
 with gradients from the hand-written adjoint (isspa_forces).  Charges are
 per topology atom, shared by every frame.
 
-    python -m isspa_amd.forcematch [--frames 256] [--steps 200] [--lr 0.01]
+    python -m isspa_amd.forcematch [--frames 256] [--steps 200] [--lr 0.01] [--lr-q 0.1]
 """
 from __future__ import annotations
 
@@ -75,8 +75,9 @@ def perturb(theta: np.ndarray, q: np.ndarray, rng, sigma=0.05, eps=0.10, dq_e=0.
 
 
 def fit(model, n_frames: int, theta0, q0, target, steps: int = 200, lr: float = 0.01,
-        log=None):
-    """Adam in relative units for theta and in 0.05 e units for the charges.
+        lr_q: float = None, cosine: bool = False, lbfgs: bool = False, log=None):
+    """Adam in relative units for theta and in 0.05 e units for the charges
+    (learning rate lr_q, default lr), optionally with a cosine decay to lr/100.
     Returns (theta, q, history)."""
     import torch
 
@@ -87,9 +88,33 @@ def fit(model, n_frames: int, theta0, q0, target, steps: int = 200, lr: float = 
     rel = torch.zeros_like(th_base, requires_grad=True)
     dq = torch.zeros_like(q_base, requires_grad=True)
     qs = 0.05 * AMBER_CHARGE
-    opt = torch.optim.Adam([rel, dq], lr=lr)
     norm = float((target ** 2).mean())
     hist = []
+    if lbfgs:
+        # reference optimiser for identifiability: L-BFGS with a strong-Wolfe
+        # line search, `steps` outer iterations
+        opt = torch.optim.LBFGS([rel, dq], lr=1.0, max_iter=20, history_size=20,
+                                line_search_fn="strong_wolfe")
+
+        def closure():
+            opt.zero_grad()
+            f = isspa_forces(th_base * (1.0 + rel), (q_base + qs * dq).repeat(n_frames), model)
+            L = ((f - target) ** 2).mean() / norm
+            L.backward()
+            hist.append(L.item())
+            return L
+
+        for it in range(steps):
+            opt.step(closure)
+            if log:
+                log(f"lbfgs iteration {it:4d}  evaluations {len(hist)}  loss {hist[-1]:.4e}")
+        with torch.no_grad():
+            return (th_base * (1.0 + rel)).cpu().numpy(), (q_base + qs * dq).cpu().numpy(), hist
+    opt = torch.optim.Adam([{"params": [rel], "lr": lr},
+                            {"params": [dq], "lr": lr if lr_q is None else lr_q}])
+    sched = None
+    if cosine:
+        sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=steps, eta_min=lr / 100)
     for it in range(steps):
         opt.zero_grad()
         th = th_base * (1.0 + rel)
@@ -98,6 +123,8 @@ def fit(model, n_frames: int, theta0, q0, target, steps: int = 200, lr: float = 
         L = ((f - target) ** 2).mean() / norm
         L.backward()
         opt.step()
+        if sched is not None:
+            sched.step()
         hist.append(L.item())
         if log and (it % 20 == 0 or it == steps - 1):
             log(f"step {it:4d}  loss {hist[-1]:.4e}")
@@ -110,6 +137,9 @@ def main(argv=None):
     ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--lr", type=float, default=0.01)
+    ap.add_argument("--lr-q", type=float, default=0.1)
+    ap.add_argument("--cosine", action="store_true")
+    ap.add_argument("--lbfgs", action="store_true", help="L-BFGS instead of Adam (diagnostic)")
     ap.add_argument("--stride", type=int, default=20)
     ap.add_argument("--seed", type=int, default=2025)
     args = ap.parse_args(argv)
@@ -138,15 +168,18 @@ def main(argv=None):
     th0, q0 = perturb(th_true, q_true, rng)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    th, q, hist = fit(model, args.frames, th0, q0, target, steps=args.steps, lr=args.lr, log=log)
+    th, q, hist = fit(model, args.frames, th0, q0, target, steps=args.steps, lr=args.lr,
+                      lr_q=args.lr_q, cosine=args.cosine, lbfgs=args.lbfgs, log=log)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
     rel_err = lambda a, b: float(np.max(np.abs(a - b) / np.abs(b)))
     out = {
         "workload": f"{args.frames} frames x {base.n_atoms} atoms, nMC={base.n_mc}",
-        "steps": args.steps, "lr": args.lr,
+        "optimizer": "lbfgs" if args.lbfgs else "adam", "evaluations": len(hist),
+        "steps": args.steps, "lr": args.lr, "lr_q": args.lr_q, "cosine": args.cosine,
         "loss_first": hist[0], "loss_last": hist[-1],
         "seconds_per_step": wall / args.steps,
+        "seconds_per_evaluation": wall / len(hist),
         "theta_max_rel_err_start": {n: rel_err(th0[:, k], th_true[:, k])
                                     for k, n in enumerate(("sigma", "eps", "x0", "g0", "alpha"))},
         "theta_max_rel_err_end": {n: rel_err(th[:, k], th_true[:, k])
