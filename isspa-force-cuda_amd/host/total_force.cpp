@@ -126,6 +126,9 @@ int main(int argc, char *argv[])
         if (graph) {
             if (write) {
                 isspa_check(md_engine_forces(eng, stream), "md_engine_forces");
+                // the engine keeps its own restraint centres of mass
+                if (configs.us == 1)
+                    isspa_check(md_engine_get_com(eng, bias.groupComPos_d, stream), "md_engine_get_com");
                 isspa_check(isspa_stream_sync(stream), "sync");
                 times.startWriteTimer();
                 atoms.get_pos_vel_for_from_gpu();
