@@ -234,6 +234,7 @@ extern "C" int isspa_diff_vjp(isspa_diff *m, const float *xyz4_d, const double *
     va.Ptot = im->P;
     va.pts = im->pts;
     va.rec4 = reinterpret_cast<const float4 *>(im->rec);
+    va.aux4 = rec_aux(im);
     va.arec = m->arec;
     va.xq = m->xq;
     va.itype = im->itype;

@@ -181,8 +181,9 @@ extern "C" int isspa_model_create_batch(const isspa_model_desc *d, const isspa_w
     if (d->n_atoms < 1 || d->n_types < 1 || d->n_mc < 1)
         return fail(ISSPA_ERR_ARG, "isspa_model_create: n_atoms=%d n_types=%d n_mc=%d must be >= 1",
                     d->n_atoms, d->n_types, d->n_mc);
-    if ((long long)d->n_atoms * d->n_mc > 0x7fffffffLL)
-        return fail(ISSPA_ERR_ARG, "isspa_model_create: n_atoms*n_mc exceeds 2^31");
+    // the force pass indexes 4 float4 per point with 32-bit integers
+    if ((long long)d->n_atoms * d->n_mc >= (1LL << 29))
+        return fail(ISSPA_ERR_ARG, "isspa_model_create: n_atoms*n_mc must be below 2^29");
     if (d->n_g < 2 || d->n_e < d->n_g || d->n_f < 2)
         return fail(ISSPA_ERR_ARG, "isspa_model_create: table rows n_g=%d n_e=%d n_f=%d (need n_g>=2, n_e>=n_g, n_f>=2)",
                     d->n_g, d->n_e, d->n_f);
@@ -232,7 +233,7 @@ extern "C" int isspa_model_create_batch(const isspa_model_desc *d, const isspa_w
     TRY(upload(&m->e, ep.data(), ep.size()));
     TRY(upload(&m->f, fp.data(), fp.size()));
     TRY(dalloc(&m->pts, m->P));
-    TRY(dalloc(&m->rec, (size_t)m->P * kRecFloats));
+    TRY(dalloc(&m->rec, (size_t)m->P * (kRecFloats + kAuxFloats)));  // records, then aux
     TRY(dalloc(&m->g_out, m->P));
     TRY(dalloc(&m->cnt_out, m->P));
     TRY(dalloc(&m->E_out, 3 * (size_t)m->P));
@@ -311,7 +312,7 @@ EpilogueArgs epilogue_args(isspa_model *m)
     ea.nmc = m->d.n_mc; ea.part_g = m->part_g; ea.part_cnt = m->part_cnt;
     ea.part_s0 = m->part_s0; ea.part_ein = m->part_ein; ea.F = m->F;
     ea.e0 = m->d.e0; ea.p0 = m->d.p0; ea.q0 = m->d.q0; ea.o0 = m->d.o0;
-    ea.rec = m->rec; ea.g_out = m->g_out; ea.cnt_out = m->cnt_out; ea.E_out = m->E_out;
+    ea.rec = m->rec; ea.aux = rec_aux(m); ea.g_out = m->g_out; ea.cnt_out = m->cnt_out; ea.E_out = m->E_out;
     ea.E0_out = m->E0_out;
     return ea;
 }
@@ -320,6 +321,7 @@ ForceArgs force_args(isspa_model *m, const float4 *x)
 {
     ForceArgs fa{};
     fa.wins = m->wins; fa.blocks = m->pblk; fa.rec4 = reinterpret_cast<const float4 *>(m->rec);
+    fa.aux4 = rec_aux(m);
     fa.xyzq = x; fa.itype = m->itype; fa.rmax2 = m->rmax2; fa.tab = m->tv; fa.nmc = m->d.n_mc;
     fa.pbc_flag = m->pbc_flag; fa.part = m->part_f; fa.C = m->C;
     return fa;

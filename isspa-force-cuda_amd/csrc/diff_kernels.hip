@@ -193,20 +193,21 @@ __global__ __launch_bounds__(kB) void k_vjp_points(VjpArgs a)
     const int j1 = min(W.a0 + W.n, j0 + W.aps);
     const bool valid = m < P;
     const long long gp = (long long)W.a0 * a.nmc + (valid ? m : 0);
-    const float4 q0 = a.rec4[4 * gp + 0];
-    const float4 q1 = a.rec4[4 * gp + 1];
-    const float4 q3 = a.rec4[4 * gp + 3];
+    const float4 q0 = a.rec4[4 * gp + 0];  // p, Ehat.x
+    const float4 q1 = a.rec4[4 * gp + 1];  // Ehat.yz
+    const float4 q3 = a.rec4[4 * gp + 3];  // A2, 3 A2, A3, w
+    const float A1 = a.aux4[gp].w;
     const bool pbc = a.pbc_flag[bt.x] != 0;
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int jt = j0; jt < j1; jt += kTile) {
         const int je = min(j1, jt + kTile);
         PointBar b{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (pbc)
-            vjp_points_tile<true>(a, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q3.x,
-                                  q3.y, q3.z, b);
+            vjp_points_tile<true>(a, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, A1,
+                                  q3.x, q3.z, b);
         else
-            vjp_points_tile<false>(a, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q3.x,
-                                   q3.y, q3.z, b);
+            vjp_points_tile<false>(a, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, A1,
+                                   q3.x, q3.z, b);
         acc[0] += b.hx; acc[1] += b.hy; acc[2] += b.hz;
         acc[3] += b.a1; acc[4] += b.a2; acc[5] += b.a3;
         acc[6] += b.cx; acc[7] += b.cy; acc[8] += b.cz;

@@ -276,9 +276,12 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     r4[0] = make_float4(p.x, p.y, p.z, (float)eh[0]);
     r4[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + cp * E0[0] / 3.0),
                         (float)(A1 * eh[1] + cp * E0[1] / 3.0));
-    r4[2] = make_float4((float)(A1 * eh[2] + cp * E0[2] / 3.0), (float)(cp * E0[0]),
-                        (float)(cp * E0[1]), (float)(cp * E0[2]));
-    r4[3] = make_float4((float)A1, (float)A2, (float)A3, (float)w);
+    r4[2] = make_float4((float)(A1 * eh[2] + cp * E0[2] / 3.0),
+                        (float)(3.0 * A1 * eh[0] + cp * E0[0]),
+                        (float)(3.0 * A1 * eh[1] + cp * E0[1]),
+                        (float)(3.0 * A1 * eh[2] + cp * E0[2]));
+    r4[3] = make_float4((float)A2, (float)(3.0 * A2), (float)A3, (float)w);
+    a.aux[gp] = make_float4((float)(cp * E0[0]), (float)(cp * E0[1]), (float)(cp * E0[2]), (float)A1);
     if (a.g_out) a.g_out[gp] = g;
     if (a.cnt_out) a.cnt_out[gp] = cnt;
     if (a.E_out)
@@ -290,15 +293,19 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
 // ---------------------------------------------------------------------------
 // A.3 force pass (src/isspa_force_cuda.cu:208-327)
 //
-// For receiver i and point m (r = p_m - x_i, u = Ehat.r/d, P2 = 7.5u^2-1.5,
-// P3 = (17.5u^2-7.5)u), the reference's three multipole terms plus the CDD
-// term, with -q_i vtot_i factored out, are
-//     S += alpha r - d^-3 V - d^-4 k Ehat
+// For receiver i and point m (r = p_m - x_i, d = |r|, u = Ehat.r/d,
+// P2 = 7.5u^2-1.5, P3 = (17.5u^2-7.5)u), the reference's three multipole
+// terms plus the CDD term, with -q_i vtot_i factored out, are
+//     S += alpha r - d^-3 V - k Ehat
 //     alpha = d^-4 (3 A1 u + d^-1 (A2 P2 + d^-1 A3 P3)) + d^-5 (E0c.r)
-//     k     = 3 u A2 + d^-1 P2 A3
-// with V = A1 Ehat + cp E0/3 and E0c = cp E0.  Inside the receiver's sphere
-// the CDD part (d^-5 (E0c.r) r - d^-3 E0c/3) is removed again and the LJ
-// table term is added instead (with -vtot_i factored out).
+//     k     = d^-4 (3 u A2 + d^-1 P2 A3)
+// with V = A1 Ehat + cp E0/3 and E0c = cp E0.  With e = Ehat.r (u = e/d),
+// t = e/d^2 and P3' = 17.5u^2-7.5, both coefficients collapse onto d^-5:
+//     alpha = d^-5 (D.r + A2 P2 + A3 P3' t),   D = 3 A1 Ehat + E0c
+//     k     = d^-5 (3 A2 e + A3 P2)
+// which is 37 VALU operations per pair (one of them v_rsq).  Inside the
+// receiver's sphere the CDD part (d^-5 (E0c.r) r - d^-3 E0c/3) is removed
+// again and the LJ table term is added instead (with -vtot_i factored out).
 // ---------------------------------------------------------------------------
 struct ForceAcc {
     double ex, ey, ez;  // electrostatic (pol + CDD) sum, or pol in split mode
@@ -306,24 +313,36 @@ struct ForceAcc {
     double cx, cy, cz;  // CDD sum (split mode only)
 };
 
+// d^-5 (E0c.r) r - d^-3 E0c/3 for one pair, E0c = c.xyz
+__device__ __forceinline__ float3 cdd_term(float4 c, float rx, float ry, float rz, float rinv3,
+                                           float rinv5)
+{
+    const float ec = __builtin_fmaf(c.z, rz, __builtin_fmaf(c.y, ry, c.x * rx));
+    const float ac = ec * rinv5;
+    const float bc = rinv3 * (1.0f / 3.0f);
+    return make_float3(__builtin_fmaf(ac, rx, -bc * c.x), __builtin_fmaf(ac, ry, -bc * c.y),
+                       __builtin_fmaf(ac, rz, -bc * c.z));
+}
+
 template <bool PBC, bool SPLIT, bool AN>
-__device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, long long m1,
+__device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__restrict__ rec,
+                                           const float4 *__restrict__ aux, int m0, int m1,
                                            float L, float h, float xi, float yi, float zi,
                                            float R2, const float *__restrict__ Ftab, int ti,
                                            float &sx,
                                            float &sy, float &sz, ForceAcc &acc)
 {
-    const float4 *__restrict__ rec = a.rec4;
+    // 32-bit point indices within the window keep the loop test on the SALU
 #pragma unroll 2
-    for (long long m = m0; m < m1; ++m) {
+    for (int m = m0; m < m1; ++m) {
         const float4 q0 = rec[4 * m + 0];  // wave-uniform
         const float4 q1 = rec[4 * m + 1];
         const float4 q2 = rec[4 * m + 2];
         const float4 q3 = rec[4 * m + 3];
         const float Ex = q0.w, Ey = q1.x, Ez = q1.y;
         const float Vx = q1.z, Vy = q1.w, Vz = q2.x;
-        const float Cx = q2.y, Cy = q2.z, Cz = q2.w;
-        const float A1 = q3.x, A2 = q3.y, A3 = q3.z, w = q3.w;
+        const float Dx = q2.y, Dy = q2.z, Dz = q2.w;
+        const float A2 = q3.x, A2x3 = q3.y, A3 = q3.z, w = q3.w;
         float rx = q0.x - xi, ry = q0.y - yi, rz = q0.z - zi;
         if (PBC) {
             rx = min_image1(rx, L, h);
@@ -332,22 +351,19 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, lon
         }
         const float d2 = dist2_f32(rx, ry, rz);
         const float rinv = rsqrt_f32(d2);
-        const float ue = __builtin_fmaf(Ez, rz, __builtin_fmaf(Ey, ry, Ex * rx));
-        const float ec = __builtin_fmaf(Cz, rz, __builtin_fmaf(Cy, ry, Cx * rx));
-        const float u = ue * rinv;
+        const float e = __builtin_fmaf(Ez, rz, __builtin_fmaf(Ey, ry, Ex * rx));
         const float rinv2 = rinv * rinv;
-        const float rinv3 = rinv2 * rinv;
-        const float rinv4 = rinv2 * rinv2;
-        const float u2 = u * u;
+        const float t = e * rinv2;
+        const float u2 = e * t;
         const float P2 = __builtin_fmaf(7.5f, u2, -1.5f);
-        const float P3 = __builtin_fmaf(17.5f, u2, -7.5f) * u;
-        const float u3 = 3.0f * u;
-        float hh = __builtin_fmaf(A2, P2, (A3 * P3) * rinv);
-        hh = __builtin_fmaf(hh, rinv, A1 * u3);
-        const float k = __builtin_fmaf(P2 * A3, rinv, u3 * A2) * rinv4;
+        const float P3p = __builtin_fmaf(17.5f, u2, -7.5f);
+        const float rinv3 = rinv2 * rinv;
+        const float rinv5 = rinv3 * rinv2;
+        const float k = __builtin_fmaf(A2x3, e, A3 * P2) * rinv5;
         const bool in = d2 <= R2;
         if (!SPLIT) {
-            const float alpha = __builtin_fmaf(ec, rinv, hh) * rinv4;
+            const float dr = __builtin_fmaf(Dz, rz, __builtin_fmaf(Dy, ry, Dx * rx));
+            const float alpha = __builtin_fmaf(A2, P2, __builtin_fmaf(A3, P3p * t, dr)) * rinv5;
             sx = __builtin_fmaf(alpha, rx, sx);
             sy = __builtin_fmaf(alpha, ry, sy);
             sz = __builtin_fmaf(alpha, rz, sz);
@@ -358,17 +374,21 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, lon
             sy = __builtin_fmaf(-k, Ey, sy);
             sz = __builtin_fmaf(-k, Ez, sz);
         } else {
-            const float ap = hh * rinv4;
+            // polarisation alone: alpha_p r - (d^-3 A1 + k) Ehat, alpha_p
+            // being alpha without the E0c.r part
+            const float4 c4 = aux[m];
+            const float A1 = c4.w;
+            const float ap =
+                __builtin_fmaf(A2, P2, __builtin_fmaf(A3, P3p * t, (3.0f * A1) * e)) * rinv5;
             const float bp = __builtin_fmaf(rinv3, A1, k);
             sx = __builtin_fmaf(ap, rx, __builtin_fmaf(-bp, Ex, sx));
             sy = __builtin_fmaf(ap, ry, __builtin_fmaf(-bp, Ey, sy));
             sz = __builtin_fmaf(ap, rz, __builtin_fmaf(-bp, Ez, sz));
             if (!in) {
-                const float ac = ec * rinv * rinv4;
-                const float bc = rinv3 * (1.0f / 3.0f);
-                acc.cx += (double)__builtin_fmaf(ac, rx, -bc * Cx);
-                acc.cy += (double)__builtin_fmaf(ac, ry, -bc * Cy);
-                acc.cz += (double)__builtin_fmaf(ac, rz, -bc * Cz);
+                const float3 c = cdd_term(c4, rx, ry, rz, rinv3, rinv5);
+                acc.cx += (double)c.x;
+                acc.cy += (double)c.y;
+                acc.cz += (double)c.z;
             }
         }
         if (in) {
@@ -389,11 +409,10 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, long long m0, lon
             acc.lz += (double)(fl * rz);
             if (!SPLIT) {
                 // remove the CDD part taken in the main path
-                const float ac = ec * rinv * rinv4;
-                const float bc = rinv3 * (1.0f / 3.0f);
-                acc.ex -= (double)__builtin_fmaf(ac, rx, -bc * Cx);
-                acc.ey -= (double)__builtin_fmaf(ac, ry, -bc * Cy);
-                acc.ez -= (double)__builtin_fmaf(ac, rz, -bc * Cz);
+                const float3 c = cdd_term(aux[m], rx, ry, rz, rinv3, rinv5);
+                acc.ex -= (double)c.x;
+                acc.ey -= (double)c.y;
+                acc.ez -= (double)c.z;
             }
         }
     }
@@ -407,8 +426,10 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const int il = bt.y * kBlock + threadIdx.x;  // receiver within the window
     const int chunk = bt.z;
     const long long pbase = (long long)W.a0 * a.nmc;
-    const long long m0 = pbase + (long long)chunk * W.ppc;
-    const long long m1 = min(pbase + (long long)W.n * a.nmc, m0 + W.ppc);
+    const float4 *__restrict__ rec = a.rec4 + 4 * pbase;  // this window's records
+    const float4 *__restrict__ aux = a.aux4 + pbase;
+    const int m0 = chunk * W.ppc;                        // window-local point range
+    const int m1 = min(W.n * a.nmc, m0 + W.ppc);
     const bool valid = il < W.n;
     const int i = W.a0 + (valid ? il : 0);
     const float4 xi = a.xyzq[i];
@@ -417,14 +438,14 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const float *Ftab = AN ? nullptr : a.tab.f + ti * a.tab.ld_f;
     const bool pbc = a.pbc_flag[bt.x] != 0;
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (long long mt = m0; mt < m1; mt += kForceTile) {
-        const long long me = min(m1, mt + kForceTile);
+    for (int mt = m0; mt < m1; mt += kForceTile) {
+        const int me = min(m1, mt + kForceTile);
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         if (pbc)
-            force_tile<true, SPLIT, AN>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx, sy,
+            force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx, sy,
                                         sz, acc);
         else
-            force_tile<false, SPLIT, AN>(a, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx,
+            force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx,
                                          sy, sz, acc);
         acc.ex += (double)sx;
         acc.ey += (double)sy;

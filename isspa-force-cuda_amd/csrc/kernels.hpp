@@ -47,9 +47,13 @@ struct TableView {
 //   [0..2]  p      MC point position
 //   [3..5]  Ehat   unit mean-field direction
 //   [6..8]  V      A1*Ehat + cp*E0/3  (E-hat-like terms of dipole + CDD)
-//   [9..11] E0c    cp*E0              (CDD r-coefficient vector)
-//   [12]    A1 = p0 <P1> w   [13] A2 = q0 <P2> w   [14] A3 = o0 <P3> w   [15] w = g/cnt
+//   [9..11] D      3 A1 Ehat + cp*E0  (r-coefficient vector of dipole + CDD)
+//   [12] A2 = q0 <P2> w   [13] 3 A2   [14] A3 = o0 <P3> w   [15] w = g/cnt
+// with A1 = p0 <P1> w.  A second array of P float4 {cp*E0, A1} (read only by
+// the in-sphere branch, the component split and the adjoint) follows the
+// records in the same allocation.
 constexpr int kRecFloats = 16;
+constexpr int kAuxFloats = 4;
 
 struct FieldArgs {
     const WinDesc *wins;
@@ -81,6 +85,7 @@ struct EpilogueArgs {
     long long F;
     float e0, p0, q0, o0;
     float *rec;         // [P][16]
+    float4 *aux;        // [P] {cp E0, A1}
     double *g_out;      // [P]      (may be null)
     int *cnt_out;       // [P]
     double *E_out;      // [P][3]
@@ -91,6 +96,7 @@ struct ForceArgs {
     const WinDesc *wins;
     const int4 *blocks;   // {w, receiver block, chunk, -}
     const float4 *rec4;   // [P][4] float4 view of the records
+    const float4 *aux4;   // [P] {cp E0, A1}
     const float4 *xyzq;
     const int *itype;
     const float *rmax2;   // per type
@@ -209,7 +215,7 @@ struct VjpArgs {
     const int4 *fblk, *pblk;   // point-lane and atom-lane block tables
     int nmc;
     long long Ptot;
-    const float4 *pts, *rec4, *arec, *xq;
+    const float4 *pts, *rec4, *aux4, *arec, *xq;
     const int *itype;
     const float *rmax2;
     const DiffType *dtypes;

@@ -32,7 +32,7 @@ struct isspa_model {
     float *rmax = nullptr, *rmax2 = nullptr, *vtot = nullptr;
     float *g = nullptr, *e = nullptr, *f = nullptr;
     float4 *pts = nullptr;
-    float *rec = nullptr;
+    float *rec = nullptr;  // [P][kRecFloats] records, then [P] float4 aux
     double *g_out = nullptr, *E_out = nullptr, *E0_out = nullptr;
     int *cnt_out = nullptr;
     int *pbc_flag = nullptr;
@@ -71,6 +71,12 @@ struct nonbond_model {
     long long B = 0;
     double *part = nullptr;
 };
+
+// the {cp E0, A1} array stored after the point records
+inline float4 *rec_aux(isspa_model *m)
+{
+    return reinterpret_cast<float4 *>(m->rec + (size_t)m->P * kRecFloats);
+}
 
 FieldArgs field_args(isspa_model *m, const float4 *x);
 EpilogueArgs epilogue_args(isspa_model *m);
