@@ -602,8 +602,7 @@ extern "C" int nonbond_model_destroy(nonbond_model *m)
     return ISSPA_OK;
 }
 
-int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
-                       double *o1, double *o2, hipStream_t s)
+int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
     HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
@@ -612,9 +611,22 @@ int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, d
     a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
     a.pbc_flag = m->pbc_flag; a.part = m->part; a.B = m->B;
     HIPCHK(launch_nonbond(a, m->nblk, split, s));
+    return ISSPA_OK;
+}
+
+int nonbond_reduce(nonbond_model *m, bool split, float4 *f, double *o0, double *o1, double *o2,
+                   hipStream_t s)
+{
     HIPCHK(launch_chunk_reduce3(m->wins, m->atom_win, m->part, m->B, m->n, split ? 9 : 3, f, o0,
                                 o1, o2, s));
     return ISSPA_OK;
+}
+
+int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
+                       double *o1, double *o2, hipStream_t s)
+{
+    CHK(nonbond_partials(m, xyzq4_d, split, s));
+    return nonbond_reduce(m, split, f, o0, o1, o2, s);
 }
 
 extern "C" int nonbond_force(nonbond_model *m, const float *xyzq4_d, float *f4_d, void *stream)
@@ -929,7 +941,14 @@ struct md_engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     unsigned captured_generation = 0;
+    // the all-pairs kernel runs on a side stream beside the IS-SPA passes
+    // (default priority; fork and join by events, which a capture records as
+    // graph edges); its chunk reduction runs after the join
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
+
+constexpr int kForkMinAtoms = 4096;
 
 static void engine_free(md_engine *e)
 {
@@ -937,6 +956,11 @@ static void engine_free(md_engine *e)
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->exec = nullptr;
     e->graph = nullptr;
+    if (e->side) (void)hipStreamDestroy(e->side);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    e->side = nullptr;
+    e->ev_fork = e->ev_join = nullptr;
     dfree(e->step_d); dfree(e->com_d); dfree(e->wins); dfree(e->atom_win);
 }
 
@@ -973,6 +997,14 @@ extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_mo
         hipError_t he = hipMemset(e->step_d, 0, sizeof(uint32_t));
         if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     }
+    // below a few thousand atoms the all-pairs kernel is a few microseconds and
+    // the fork and join would cost more than the overlap saves
+    if (rc == ISSPA_OK && isspa && nb && n >= kForkMinAtoms) {
+        hipError_t he = hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming);
+        if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "side stream: %s", hipGetErrorString(he));
+    }
     if (rc != ISSPA_OK) {
         engine_free(e);
         delete e;
@@ -999,14 +1031,28 @@ extern "C" int md_engine_set_step(md_engine *e, uint32_t step, void *stream)
 
 static int engine_forces(md_engine *e, hipStream_t s)
 {
+    float4 *f4 = reinterpret_cast<float4 *>(e->f);
     HIPCHK(launch_zero(e->f, (size_t)e->n * sizeof(float4), s));
+    const bool fork = e->side != nullptr;
+    if (fork) {
+        // all-pairs partials beside the IS-SPA passes: they read x and write
+        // only the nonbond model's own buffers
+        HIPCHK(hipEventRecord(e->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(e->side, e->ev_fork, 0));
+        CHK(nonbond_partials(e->nb, e->x, false, e->side));
+        HIPCHK(hipEventRecord(e->ev_join, e->side));
+    }
     if (e->bonded) CHK(bonded_force(e->bonded, ISSPA_TERM_ALL, e->x, e->f, nullptr, s));
     if (e->isspa) {
         CHK(isspa_field_pass(e->isspa, e->x, 0u, e->step_d, s));
-        CHK(isspa_force_pass(e->isspa, e->x, false, reinterpret_cast<float4 *>(e->f), nullptr,
-                             nullptr, nullptr, s));
+        CHK(isspa_force_pass(e->isspa, e->x, false, f4, nullptr, nullptr, nullptr, s));
     }
-    if (e->nb) CHK(nonbond_force(e->nb, e->x, e->f, s));
+    if (fork) {
+        HIPCHK(hipStreamWaitEvent(s, e->ev_join, 0));
+        CHK(nonbond_reduce(e->nb, false, f4, nullptr, nullptr, nullptr, s));
+    } else if (e->nb) {
+        CHK(nonbond_force(e->nb, e->x, e->f, s));
+    }
     if (e->us) CHK(us_force(e->us, e->x, e->f, e->com_d, s));
     return ISSPA_OK;
 }

@@ -85,3 +85,7 @@ ForceArgs force_args(isspa_model *m, const float4 *x);
 // otherwise f (float4, +=) or, with f null, o0 (double[N*3], +=)
 int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
                 double *o1, double *o2, hipStream_t s);
+// the same in two halves: per-chunk partials, then their fixed-order sum
+int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s);
+int nonbond_reduce(nonbond_model *m, bool split, float4 *f, double *o0, double *o1, double *o2,
+                   hipStream_t s);

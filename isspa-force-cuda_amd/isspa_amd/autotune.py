@@ -14,8 +14,12 @@ back-to-back launches of each stage) and keeps the fastest.
 Two candidate families:
   * target wave counts (the automatic per-window choice at several targets),
     the only sensible knob for batches of uneven windows;
-  * explicit (splits, chunks) pairs around the automatic choice, for a single
-    system.
+  * explicit (splits, chunks) pairs for a single system: a x0.5/x1/x2 grid
+    around the tiling of the best wave target, plus the pair of the best split count and
+    the best chunk count found separately.  The field pass and its epilogue
+    depend only on the split count and the force pass and its reduction only
+    on the chunk count, so each is searched on its own stages, over counts
+    whose grid fills at least 85% of its last round of wave slots.
 The choice is deterministic given the timings; ties keep the earlier
 candidate.  Changing the tiling rebuilds the launch tables, so tune before
 capturing a graph.
@@ -23,9 +27,14 @@ capturing a graph.
 from __future__ import annotations
 
 import dataclasses
+import math
 from typing import List, Optional, Sequence, Tuple
 
 PASS_STAGES = ("field", "epilogue", "force", "reduce")
+FIELD_STAGES = ("field", "epilogue")
+FORCE_STAGES = ("force", "reduce")
+WAVE_SLOTS = 8192        # 256 CUs x 4 SIMDs x 8 resident waves
+WAVES_PER_BLOCK = 4      # 256-thread blocks
 
 
 @dataclasses.dataclass
@@ -36,9 +45,39 @@ class TuneResult:
     table: List[Tuple[str, Tuple[int, ...], float]]
 
 
-def _measure(g, reps: int) -> float:
+def _measure(g, reps: int, stages=PASS_STAGES) -> float:
     prof = g.profile(reps)
-    return sum(prof[k] for k in PASS_STAGES)
+    return sum(prof[k] for k in stages)
+
+
+def effective_pieces(n: int, pieces: int) -> int:
+    """The piece count the library uses for `pieces` requested pieces of `n`
+    items: pieces of ceil(n / pieces) items (isspa_model_set_tiling)."""
+    pieces = max(1, min(pieces, n))
+    per = -(-n // pieces)
+    return -(-n // per)
+
+
+def round_fill(blocks: int, pieces: int) -> float:
+    """Fraction of the last round of wave slots that a grid of blocks x pieces
+    256-thread blocks fills (1.0 = whole rounds)."""
+    rounds = blocks * pieces * WAVES_PER_BLOCK / WAVE_SLOTS
+    last = rounds - math.floor(rounds)
+    return 1.0 if last == 0.0 else last
+
+
+def round_aligned(n: int, blocks: int, base: int, lo: float = 0.5, hi: float = 2.0,
+                  min_fill: float = 0.85, limit: int = 6) -> List[int]:
+    """Piece counts of `n` items between lo*base and hi*base whose grid
+    (`blocks` blocks per piece) fills at least `min_fill` of its last round,
+    the `limit` nearest to `base`, plus `base` itself."""
+    found = set()
+    for p in range(max(1, int(base * lo)), max(2, int(base * hi)) + 1):
+        pe = effective_pieces(n, p)
+        if round_fill(blocks, pe) >= min_fill:
+            found.add(pe)
+    picked = sorted(found, key=lambda p: (abs(p - base), p))[:limit]
+    return sorted(set(picked) | {effective_pieces(n, base)})
 
 
 def tune(g, reps: int = 3, waves: Sequence[int] = (8192, 16384, 24576, 32768, 49152, 65536,
@@ -56,18 +95,39 @@ def tune(g, reps: int = 3, waves: Sequence[int] = (8192, 16384, 24576, 32768, 49
         if log:
             log(f"autotune waves={wv}: {ms:.4f} ms  tiling {g.tiling()}")
     if tilings is None and g.windows is None:
+        # centre the explicit candidates on the best wave target's tiling
+        best_wv = min(table, key=lambda t: t[2])[1][0] if table else 32768
         g.set_tiling(0, 0)
-        g.set_target_waves(32768)
+        g.set_target_waves(best_wv)
         s0, c0 = g.tiling()
-        tilings = sorted({(max(1, int(s0 * a)), max(1, int(c0 * b)))
-                          for a in (0.5, 1.0, 2.0) for b in (0.5, 1.0, 2.0)})
+        grid = {(max(1, int(s0 * a)), max(1, int(c0 * b)))
+                for a in (0.5, 1.0, 2.0) for b in (0.5, 1.0, 2.0)}
+        n, P = g.system.n_atoms, g.system.n_points
+        best = []
+        for kind, cands, stages, pick in (
+                ("field-splits", round_aligned(n, -(-P // 256), s0), FIELD_STAGES,
+                 lambda v: (v, c0)),
+                ("force-chunks", round_aligned(P, -(-n // 256), c0), FORCE_STAGES,
+                 lambda v: (s0, v))):
+            stage_ms = []
+            for v in cands:
+                g.set_tiling(*pick(v))
+                ms = _measure(g, reps, stages)
+                stage_ms.append((ms, v))
+                table.append((kind, (v,), ms))
+                if log:
+                    log(f"autotune {kind}={v}: {ms:.4f} ms ({' + '.join(stages)})")
+            best.append(min(stage_ms)[1])
+        grid.add(tuple(best))
+        tilings = sorted(grid)
     for sc in tilings or ():
         g.set_tiling(*sc)
         ms = _measure(g, reps)
         table.append(("tiling", tuple(sc), ms))
         if log:
             log(f"autotune tiling={sc}: {ms:.4f} ms")
-    best = min(range(len(table)), key=lambda k: (table[k][2], k))
+    whole = [k for k in range(len(table)) if table[k][0] in ("waves", "tiling")]
+    best = min(whole, key=lambda k: (table[k][2], k))
     kind, value, ms = table[best]
     if kind == "waves":
         g.set_tiling(0, 0)

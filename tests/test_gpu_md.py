@@ -161,3 +161,37 @@ def test_graph_replay_matches_eager(cfg1):
         assert np.array_equal(eager.v.cpu().numpy(), graph.v.cpu().numpy())
     eager.close()
     graph.close()
+
+
+def test_engine_side_stream_path():
+    """At 4,400 atoms the engine runs the all-pairs kernel on a side stream
+    beside the IS-SPA passes.  Its forces equal the separate calls made in the
+    same order bit for bit, and two replays of the captured step equal two
+    eager steps bit for bit."""
+    from isspa_amd import synth
+    s = synth.make_lattice(200, 120.0, 6, 200.0, seed=91)
+    s.n_mc = 2
+    assert s.n_atoms >= 4096
+    print(f"N={s.n_atoms} nMC={s.n_mc}", flush=True)
+    g = _gpu(s)
+    g.set_step(3)
+    g.forces()
+    torch.cuda.synchronize()
+    got = g.f.clone()
+    g.zero_forces()
+    g.bonded_force()
+    g.isspa_force(step=3)
+    g.nonbond_force()
+    torch.cuda.synchronize()
+    assert torch.equal(got, g.f)
+    g.close()
+    eager, graph = _gpu(s), _gpu(s)
+    eager.set_step(0)
+    graph.set_step(0)
+    graph.capture(1)
+    eager.step(2)
+    graph.replay(2)
+    torch.cuda.synchronize()
+    assert torch.equal(eager.x, graph.x) and torch.equal(eager.v, graph.v)
+    eager.close()
+    graph.close()
