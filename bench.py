@@ -1,6 +1,6 @@
 """IS-SPA MD throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--cpu-threads T] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2] [--cpu-threads T] [--no-cpu]
 
 A step is one full MD step of the benchmark workload -- bonded terms, IS-SPA
 MC points + field + force passes, all-pairs nonbond, leap-frog + Andersen --
@@ -98,8 +98,9 @@ def _time_replays(launch, steps):
     return time.perf_counter() - t0
 
 
-def batched_bench(steps, plain_rounds, warmup, tune_reps):
-    """Batched 64-window step vs the plain per-window loop on the same GPU."""
+def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
+    """Batched 64-window step vs the plain per-window loop on the same GPU.
+    Both capture `per_graph` consecutive steps in each graph."""
     import numpy as np
     import torch
     from isspa_amd import autotune, batch, capi
@@ -111,8 +112,11 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps):
     g = GpuSystem(merged, "cuda:0")
     tr = autotune.tune(g, reps=tune_reps, waves=(1024, 2048, 4096, 8192, 16384, 32768), log=log)
     log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per IS-SPA pass")
+    K = max(1, int(per_graph))
+    steps = max(K, steps - steps % K)
+    plain_rounds = max(K, plain_rounds - plain_rounds % K)
     g.set_step(0)
-    g.capture(1)
+    g.capture(K)
     st = torch.cuda.Stream()
     eng = g.make_engine()
 
@@ -120,15 +124,15 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps):
         capi.check(capi.lib.md_engine_replay(eng, 1, st.cuda_stream))
 
     with torch.cuda.stream(st):
-        _time_replays(run_batch, warmup)
-        t_b = _time_replays(run_batch, steps)
+        _time_replays(run_batch, max(1, warmup // K))
+        t_b = _time_replays(run_batch, steps // K)
     if not np.all(np.isfinite(g.x.cpu().numpy())):
         raise RuntimeError("batched run produced non-finite positions")
     plain = []
     for w, s in enumerate(windows):
         p = GpuSystem(s, "cuda:0", system_id=w)
         p.set_step(0)
-        p.capture(1)
+        p.capture(K)
         plain.append((p, p.make_engine()))
 
     def run_plain():
@@ -136,8 +140,8 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps):
             capi.check(capi.lib.md_engine_replay(e, 1, st.cuda_stream))
 
     with torch.cuda.stream(st):
-        _time_replays(run_plain, max(1, warmup // 10))
-        t_p = _time_replays(run_plain, plain_rounds)
+        _time_replays(run_plain, max(1, warmup // (10 * K)))
+        t_p = _time_replays(run_plain, plain_rounds // K)
     ms_b = 1e3 * t_b / steps
     ms_p = 1e3 * t_p / plain_rounds
     nsd_b = W * 0.1728 * steps / t_b
@@ -150,6 +154,7 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps):
                     f"{kinds.count('chain')} x 22-atom chain, {kinds.count('big')} x 220-atom "
                     f"(10 chains); k=10 distance restraint, centres 3-15 A; nMC={merged.n_mc}",
         "windows": W, "atoms": merged.n_atoms, "steps": steps, "plain_rounds": plain_rounds,
+        "steps_per_graph": K,
         "ms_per_step": round(ms_b, 5),
         "window_ns_per_day": round(nsd_b, 2),
         "plain_loop_ms_per_step": round(ms_p, 5),
@@ -169,7 +174,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--workload", type=int, default=2,
+                    help="plain workload index: 0 one chain, 1 1,100 atoms, 2 19,998 atoms")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-reps", type=int, default=5)
@@ -177,6 +183,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--plain-rounds", type=int, default=100)
+    ap.add_argument("--steps-per-graph", type=int, default=1,
+                    help="steps captured per graph in the batched comparison (both sides)")
     ap.add_argument("--tiling", default="", help="explicit field splits,force chunks (no tuning)")
     ap.add_argument("--target-waves", type=int, default=0, help="automatic tiling at this target")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -194,8 +202,8 @@ def main():
     from isspa_amd.engine import GpuSystem
 
     t0 = time.perf_counter()
-    s = synth.config(args.config)
-    log(f"workload config[{args.config}]: N={s.n_atoms} nMC={s.n_mc} P={s.n_points} "
+    s = synth.workload(args.workload)
+    log(f"workload {s.name}: N={s.n_atoms} nMC={s.n_mc} P={s.n_points} "
         f"box={s.lbox} built in {time.perf_counter() - t0:.1f}s")
     g = GpuSystem(s, "cuda:0")
     tune = None
@@ -261,7 +269,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"config[{args.config}]: {N // 22} seeded copies of the 22-atom chain "
+            "workload": f"{s.name}: {N // 22} seeded copies of the 22-atom chain "
                         f"({N} atoms), nMC={s.n_mc}, box {s.lbox:g} A; bonded + IS-SPA + "
                         "all-pairs nonbond + leap-frog/Andersen, one hipGraph replay per step",
             "n_atoms": N, "n_mc": s.n_mc, "mc_points": P, "dt_fs": 2.0,
@@ -289,8 +297,8 @@ def main():
     # HBM-side bytes per k_force launch from the committed FETCH_SIZE /
     # WRITE_SIZE passes (copy-calibrated); they depend on the tiling, which is
     # reported with them
-    tr_path = os.path.join(ROOT, "profiles", "k_force_config2_traffic.json")
-    if args.config == 2 and os.path.exists(tr_path):
+    tr_path = os.path.join(ROOT, "profiles", "k_force_19998atoms_nmc20_traffic.json")
+    if args.workload == 2 and os.path.exists(tr_path):
         with open(tr_path) as fh:
             tr = json.load(fh)
         out["roofline"]["traffic"] = tr["traffic_bytes"]
@@ -299,7 +307,7 @@ def main():
     g.close()
     if not args.no_batch:
         out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,
-                                       tune_reps=3)
+                                       tune_reps=3, per_graph=args.steps_per_graph)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         res = cpu_baseline(s, threads)

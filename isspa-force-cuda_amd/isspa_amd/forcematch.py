@@ -151,7 +151,7 @@ def main(argv=None):
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
-    base = synth.config(1)
+    base = synth.workload(1)
     prm = DiffParams.from_system(base)
     t0 = time.perf_counter()
     frames = sample_frames(base, args.frames, stride=args.stride)

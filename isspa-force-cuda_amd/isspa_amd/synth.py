@@ -93,7 +93,7 @@ def _tables(sig, eps, rng):
 
 
 def make_chain(seed: int = SPEC["chain_seed"]) -> System:
-    """The 22-atom chain (config[0] topology), centred at the origin."""
+    """The 22-atom chain, centred at the origin."""
     S = SPEC
     rng = np.random.default_rng(seed)
     atoms, bonds, backbone = chain_topology()
@@ -297,10 +297,13 @@ def maxwell_velocities(system: System, seed: int) -> None:
     system.vel[:, :3] = (rng.normal(size=(system.n_atoms, 3)) * s[:, None]).astype(np.float32)
 
 
-def config(index: int) -> System:
-    """Workloads of the benchmark configuration list:
+WORKLOAD_NAMES = ("chain22_nmc10", "chains50_1100atoms_nmc20", "chains909_19998atoms_nmc20")
+
+
+def workload(index: int) -> System:
+    """The plain workloads, by index:
     0: one chain, nMC=10, box 200 A;
-    1: 50 chains on a 4^3 lattice in a 60 A region, nMC=20, box 200 A;
+    1: 50 chains (1,100 atoms) on a 4^3 lattice in a 60 A region, nMC=20, box 200 A;
     2: 909 chains (19,998 atoms) on a 10^3 lattice in a 160 A region, nMC=20,
        box 400 A (the minimum image never fires)."""
     if index == 0:
@@ -317,7 +320,7 @@ def config(index: int) -> System:
         raise ValueError(f"no plain workload {index}")
     s.temperature_k, s.pnu, s.seed = 298.0, 0.01, 12345 + index
     maxwell_velocities(s, seed=100 + index)
-    s.name = f"config{index}"
+    s.name = WORKLOAD_NAMES[index]
     s.check()
     return s
 

@@ -28,13 +28,13 @@ def adi2():
 @pytest.fixture(scope="session")
 def cfg0():
     from isspa_amd import synth
-    return synth.config(0)
+    return synth.workload(0)
 
 
 @pytest.fixture(scope="session")
 def cfg1():
     from isspa_amd import synth
-    return synth.config(1)
+    return synth.workload(1)
 
 
 def gpu_available():

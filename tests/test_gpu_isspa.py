@@ -95,7 +95,7 @@ def test_isspa_large_sampled():
     the field on 2,000 sampled points, and the force on 48 sampled receivers
     computed by the oracle from the GPU's own field."""
     from isspa_amd import synth
-    s = synth.config(2)
+    s = synth.workload(2)
     g = _gpu(s, nonbond=False, bonded=False)
     comp = g.isspa_components(step=0)
     fld = {k: v.cpu().numpy() for k, v in g.field().items()}

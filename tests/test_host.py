@@ -96,7 +96,7 @@ def test_chain_topology_counts():
 
 
 def test_workloads_deterministic_and_sized():
-    a, b = synth.config(1), synth.config(1)
+    a, b = synth.workload(1), synth.workload(1)
     assert np.array_equal(a.xyzq, b.xyzq) and np.array_equal(a.vel, b.vel)
     assert a.n_atoms == 1100 and a.n_mc == 20
     ext = a.xyzq[:, :3].max(0) - a.xyzq[:, :3].min(0)
