@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=5)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched comparison and print its JSON object")
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--plain-rounds", type=int, default=100)
     ap.add_argument("--steps-per-graph", type=int, default=1,
@@ -195,6 +197,12 @@ def main():
               "(the per-GPU shard design is documented, no multi-GPU path runs here)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_only:
+        print(json.dumps({"batched": batched_bench(args.batch_steps, args.plain_rounds,
+                                                   args.warmup, tune_reps=3,
+                                                   per_graph=args.steps_per_graph)}))
+        return
 
     import numpy as np
     import torch

@@ -245,6 +245,7 @@ extern "C" int isspa_model_create_batch(const isspa_model_desc *d, const isspa_w
     tv.g = m->g; tv.e = m->e; tv.f = m->f;
     tv.n_g = d->n_g; tv.n_e = d->n_e; tv.n_f = d->n_f;
     tv.ld_g = d->n_g + 1; tv.ld_e = d->n_e + 1; tv.ld_f = d->n_f + 1;
+    tv.ntypes = d->n_types;
     tv.g_r0 = d->g_r0; tv.g_dr = d->g_dr; tv.g_inv = 1.0f / d->g_dr;
     tv.e_r0 = d->e_r0; tv.e_dr = d->e_dr; tv.e_inv = 1.0f / d->e_dr;
     tv.f_r0 = d->f_r0; tv.f_dr = d->f_dr; tv.f_inv = 1.0f / d->f_dr;

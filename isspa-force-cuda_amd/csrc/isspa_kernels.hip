@@ -31,6 +31,21 @@ namespace isspa {
 constexpr int kBlock = 256;
 constexpr int kFieldTile = 128;
 constexpr int kForceTile = 64;
+// per-type tables up to this many floats (16 KB) are staged in LDS for the
+// in-sphere branches; 8 resident blocks per CU then hold 128 of its 160 KB
+constexpr int kTabLdsFloats = 4096;
+
+__host__ __device__ inline int field_tab_floats(const TableView &t)
+{
+    return t.ntypes * (t.ld_g + t.ld_e);
+}
+__host__ __device__ inline int force_tab_floats(const TableView &t) { return t.ntypes * t.ld_f; }
+
+// copy n floats into LDS with the whole block (n is block-uniform)
+__device__ __forceinline__ void stage_lds(float *dst, const float *__restrict__ src, int n)
+{
+    for (int t = threadIdx.x; t < n; t += kBlock) dst[t] = src[t];
+}
 
 // x + R * (v 2^-24) with two roundings, never fused into one FMA, so the
 // oracle forms the same bits (v 2^-24 itself is exact)
@@ -85,13 +100,16 @@ __global__ __launch_bounds__(kBlock) void k_mc_points(
 // ---------------------------------------------------------------------------
 // A.2 field pass (src/isspa_force_cuda.cu:110-206)
 // ---------------------------------------------------------------------------
+// in-sphere state of one point: g product and count over the split, and the
+// E_in sum of the current tile (flushed into FP64 per tile)
 struct FieldRare {
     float g;
     int cnt;
-    double ex, ey, ez;
+    float ex, ey, ez;
 };
 
-__device__ __forceinline__ void field_in_sphere(const TableView &tb, int tj, float d2, float inv,
+__device__ __forceinline__ void field_in_sphere(const TableView &tb, const float *Gtab,
+                                                const float *Etab, int tj, float d2, float inv,
                                                 float s, float rx, float ry, float rz, float e0,
                                                 FieldRare &st)
 {
@@ -103,18 +121,18 @@ __device__ __forceinline__ void field_in_sphere(const TableView &tb, int tj, flo
         fac = 0.0f;
     } else if (b < tb.n_g - 1) {
         const float ph = (d - (tb.g_r0 + (float)b * tb.g_dr)) * tb.g_inv;
-        const float *G = tb.g + tj * tb.ld_g;
+        const float *G = Gtab + tj * tb.ld_g;
         fac = __builtin_fmaf(G[b + 1], ph, G[b] * (1.0f - ph));
         const float phe = (d - (tb.e_r0 + (float)b * tb.e_dr)) * tb.e_inv;
-        const float *Et = tb.e + tj * tb.ld_e;
+        const float *Et = Etab + tj * tb.ld_e;
         et = __builtin_fmaf(Et[b + 1], phe, Et[b] * (1.0f - phe));
     }
     st.g *= fac;
     st.cnt += 1;
     const float c = __builtin_fmaf(-e0, s, et * inv);
-    st.ex += (double)(c * rx);
-    st.ey += (double)(c * ry);
-    st.ez += (double)(c * rz);
+    st.ex = __builtin_fmaf(c, rx, st.ex);
+    st.ey = __builtin_fmaf(c, ry, st.ey);
+    st.ez = __builtin_fmaf(c, rz, st.ez);
 }
 
 // the same with the analytic g and E of the differentiable model
@@ -126,13 +144,14 @@ __device__ __forceinline__ void field_in_sphere_an(const DiffType &T, float d2, 
     st.g *= diff_g(T, d);
     st.cnt += 1;
     const float c = __builtin_fmaf(-e0, s, diff_E(T, d) * inv);
-    st.ex += (double)(c * rx);
-    st.ey += (double)(c * ry);
-    st.ez += (double)(c * rz);
+    st.ex = __builtin_fmaf(c, rx, st.ex);
+    st.ey = __builtin_fmaf(c, ry, st.ey);
+    st.ez = __builtin_fmaf(c, rz, st.ez);
 }
 
 template <bool PBC, bool AN>
-__device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, float L, float h,
+__device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab,
+                                           const float *Etab, int j0, int j1, float L, float h,
                                            float px, float py, float pz, float R2, float &ax,
                                            float &ay, float &az, FieldRare &st)
 {
@@ -158,7 +177,8 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, int j0, int j1, f
             if (AN)
                 field_in_sphere_an(a.dtypes[a.itype[j]], d2, inv, s, rx, ry, rz, a.e0, st);
             else
-                field_in_sphere(a.tab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0, st);
+                field_in_sphere(a.tab, Gtab, Etab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0,
+                                st);
         }
     }
 }
@@ -177,18 +197,35 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     const long long gp = (long long)W.a0 * a.nmc + m;
     const float4 p = valid ? a.pts[gp] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     const bool pbc = a.pbc_flag[bt.x] != 0;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    FieldRare st{1.0f, 0, 0.0, 0.0, 0.0};
+    // g and E tables for the in-sphere branch: LDS when they fit
+    extern __shared__ float lds_tab[];
+    const float *Gtab = a.tab.g, *Etab = a.tab.e;
+    if (!AN && field_tab_floats(a.tab) <= kTabLdsFloats) {
+        const int ng = a.tab.ntypes * a.tab.ld_g;
+        stage_lds(lds_tab, a.tab.g, ng);
+        stage_lds(lds_tab + ng, a.tab.e, a.tab.ntypes * a.tab.ld_e);
+        __syncthreads();
+        Gtab = lds_tab;
+        Etab = lds_tab + ng;
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+    FieldRare st{1.0f, 0, 0.0f, 0.0f, 0.0f};
     for (int jt = j0; jt < j1; jt += kFieldTile) {
         const int je = min(j1, jt + kFieldTile);
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        st.ex = st.ey = st.ez = 0.0f;
         if (pbc)
-            field_tile<true, AN>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
+            field_tile<true, AN>(a, Gtab, Etab, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az,
+                                 st);
         else
-            field_tile<false, AN>(a, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az, st);
+            field_tile<false, AN>(a, Gtab, Etab, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay,
+                                  az, st);
         sx += (double)ax;
         sy += (double)ay;
         sz += (double)az;
+        ex += (double)st.ex;
+        ey += (double)st.ey;
+        ez += (double)st.ez;
     }
     if (!valid) return;
     const long long o = W.fpo + (long long)split * P + m;
@@ -197,9 +234,9 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     a.part_s0[o] = sx;
     a.part_s0[a.F + o] = sy;
     a.part_s0[2 * a.F + o] = sz;
-    a.part_ein[o] = st.ex;
-    a.part_ein[a.F + o] = st.ey;
-    a.part_ein[2 * a.F + o] = st.ez;
+    a.part_ein[o] = ex;
+    a.part_ein[a.F + o] = ey;
+    a.part_ein[2 * a.F + o] = ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -324,14 +361,21 @@ __device__ __forceinline__ float3 cdd_term(float4 c, float rx, float ry, float r
                        __builtin_fmaf(ac, rz, -bc * c.z));
 }
 
+// FP32 sums of one tile: the electrostatic (or, split, polarisation) sum s and
+// the in-sphere LJ sum l; flushed into the FP64 ForceAcc per tile
+struct TileSums {
+    float sx, sy, sz;
+    float lx, ly, lz;
+};
+
 template <bool PBC, bool SPLIT, bool AN>
 __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__restrict__ rec,
                                            const float4 *__restrict__ aux, int m0, int m1,
                                            float L, float h, float xi, float yi, float zi,
-                                           float R2, const float *__restrict__ Ftab, int ti,
-                                           float &sx,
-                                           float &sy, float &sz, ForceAcc &acc)
+                                           float R2, const float *Ftab, int ti, TileSums &ts,
+                                           ForceAcc &acc)
 {
+    float &sx = ts.sx, &sy = ts.sy, &sz = ts.sz;
     // 32-bit point indices within the window keep the loop test on the SALU
 #pragma unroll 2
     for (int m = m0; m < m1; ++m) {
@@ -404,15 +448,15 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
                 }
             }
             const float fl = fs * rinv;
-            acc.lx += (double)(fl * rx);
-            acc.ly += (double)(fl * ry);
-            acc.lz += (double)(fl * rz);
+            ts.lx = __builtin_fmaf(fl, rx, ts.lx);
+            ts.ly = __builtin_fmaf(fl, ry, ts.ly);
+            ts.lz = __builtin_fmaf(fl, rz, ts.lz);
             if (!SPLIT) {
                 // remove the CDD part taken in the main path
                 const float3 c = cdd_term(aux[m], rx, ry, rz, rinv3, rinv5);
-                acc.ex -= (double)c.x;
-                acc.ey -= (double)c.y;
-                acc.ez -= (double)c.z;
+                sx -= c.x;
+                sy -= c.y;
+                sz -= c.z;
             }
         }
     }
@@ -435,21 +479,35 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const float4 xi = a.xyzq[i];
     const int ti = a.itype[i];
     const float R2 = valid ? a.rmax2[ti] : -1.0f;
-    const float *Ftab = AN ? nullptr : a.tab.f + ti * a.tab.ld_f;
+    // the receiver type's F table for the in-sphere branch: LDS when it fits
+    extern __shared__ float lds_tab[];
+    const float *Ftab = nullptr;
+    if (!AN) {
+        const float *ftab = a.tab.f;
+        if (force_tab_floats(a.tab) <= kTabLdsFloats) {
+            stage_lds(lds_tab, a.tab.f, force_tab_floats(a.tab));
+            __syncthreads();
+            ftab = lds_tab;
+        }
+        Ftab = ftab + ti * a.tab.ld_f;
+    }
     const bool pbc = a.pbc_flag[bt.x] != 0;
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int mt = m0; mt < m1; mt += kForceTile) {
         const int me = min(m1, mt + kForceTile);
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        TileSums ts{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (pbc)
-            force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx, sy,
-                                        sz, acc);
+            force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
+                                        ti, ts, acc);
         else
-            force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab, ti, sx,
-                                         sy, sz, acc);
-        acc.ex += (double)sx;
-        acc.ey += (double)sy;
-        acc.ez += (double)sz;
+            force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
+                                         ti, ts, acc);
+        acc.ex += (double)ts.sx;
+        acc.ey += (double)ts.sy;
+        acc.ez += (double)ts.sz;
+        acc.lx += (double)ts.lx;
+        acc.ly += (double)ts.ly;
+        acc.lz += (double)ts.lz;
     }
     if (!valid) return;
     const long long o = W.ppo + (long long)chunk * W.n + il;
@@ -519,10 +577,13 @@ hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const floa
 hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    if (a.dtypes)
+    if (a.dtypes) {
         hipLaunchKernelGGL(k_field<true>, dim3(nblocks), dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_field<false>, dim3(nblocks), dim3(kBlock), 0, s, a);
+    } else {
+        const int nt = field_tab_floats(a.tab);
+        const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
+        hipLaunchKernelGGL(k_field<false>, dim3(nblocks), dim3(kBlock), lds, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -542,10 +603,12 @@ hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t
         else
             hipLaunchKernelGGL((k_force<false, true>), dim3(nblocks), dim3(kBlock), 0, s, a);
     } else {
+        const int nt = force_tab_floats(a.tab);
+        const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
         if (split)
-            hipLaunchKernelGGL((k_force<true, false>), dim3(nblocks), dim3(kBlock), 0, s, a);
+            hipLaunchKernelGGL((k_force<true, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
         else
-            hipLaunchKernelGGL((k_force<false, false>), dim3(nblocks), dim3(kBlock), 0, s, a);
+            hipLaunchKernelGGL((k_force<false, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
     }
     return hipGetLastError();
 }

@@ -37,6 +37,7 @@ struct TableView {
     const float *g, *e, *f;
     int n_g, n_e, n_f;     // logical rows
     int ld_g, ld_e, ld_f;  // padded rows per type
+    int ntypes;
     float g_r0, g_dr, g_inv;
     float e_r0, e_dr, e_inv;
     float f_r0, f_dr, f_inv;
