@@ -110,7 +110,8 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
     merged = batch.merge_windows(windows)
     W = len(windows)
     g = GpuSystem(merged, "cuda:0")
-    tr = autotune.tune(g, reps=tune_reps, waves=(1024, 2048, 4096, 8192, 16384, 32768), log=log)
+    tr = autotune.tune(g, reps=tune_reps, waves=(2048, 4096, 8192, 16384, 32768, 65536, 131072),
+                       log=log)
     log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per IS-SPA pass")
     K = max(1, int(per_graph))
     steps = max(K, steps - steps % K)

@@ -71,6 +71,13 @@ static WinDesc base_desc(const Windows &L, int w)
 
 // pieces for `lanes` lanes doing `work` items each, at most `target` waves in
 // total, with at least `min_work` items per piece
+// Smallest automatic pieces: a field split keeps at least kMinSplitAtoms atoms
+// and a force chunk at least kMinChunkPoints points.  Below these the FP64
+// partial traffic and the reduction outgrow the gain in parallelism; small
+// windows of a batch need pieces this fine to fill the device.
+constexpr long long kMinSplitAtoms = 8;
+constexpr long long kMinChunkPoints = 16;
+
 static int auto_pieces(long long lanes, long long work, long long min_work, double target)
 {
     const long long lane_blocks = std::max(1LL, (lanes + 255) / 256);
@@ -109,11 +116,11 @@ static int isspa_tile(isspa_model *m)
         WinDesc d = base_desc(L, w);
         const long long n = L.n[w], P = n * nmc;
         const double tgt = std::max(1.0, m->target_waves * ((double)n * P / work_tot));
-        int s = m->user_splits > 0 ? m->user_splits : auto_pieces(P, n, 32, tgt);
+        int s = m->user_splits > 0 ? m->user_splits : auto_pieces(P, n, kMinSplitAtoms, tgt);
         s = (int)std::max(1LL, std::min((long long)s, n));
         d.aps = (int)((n + s - 1) / s);
         d.splits = (int)((n + d.aps - 1) / d.aps);
-        int c = m->user_chunks > 0 ? m->user_chunks : auto_pieces(n, P, 128, tgt);
+        int c = m->user_chunks > 0 ? m->user_chunks : auto_pieces(n, P, kMinChunkPoints, tgt);
         c = (int)std::max(1LL, std::min((long long)c, P));
         d.ppc = (int)((P + c - 1) / c);
         d.chunks = (int)((P + d.ppc - 1) / d.ppc);

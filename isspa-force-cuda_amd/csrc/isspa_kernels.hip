@@ -525,20 +525,39 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     }
 }
 
-// F_i = -q_i vtot_i S_e - vtot_i S_lj, summed over chunks in chunk order.
+// F_i = -q_i vtot_i S_e - vtot_i S_lj over the chunk partials.  A block owns
+// kRedAtoms atoms and kRedSlices chunk slices: slice t sums chunks t, t + S,
+// t + 2S, ... in order, then slice 0 adds slices 1..S-1 in order, so the sum
+// is in a fixed order independent of scheduling.  The slices give the kernel
+// S times the waves of one lane per atom, which small systems need to hide
+// the load latency.
+constexpr int kRedAtoms = 32;
+constexpr int kRedSlices = kBlock / kRedAtoms;
+
 template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.ntot) return;
-    const WinDesc W = a.wins[a.atom_win[i]];
-    const int il = i - W.a0;
-    const int comps = SPLIT ? 9 : 6;
-    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < W.chunks; ++c) {
-        const long long o = W.ppo + (long long)c * W.n + il;
-        for (int k = 0; k < comps; ++k) s[k] += a.part[k * a.C + o];
+    constexpr int comps = SPLIT ? 9 : 6;
+    __shared__ double red[kRedSlices - 1][comps][kRedAtoms];
+    const int al = threadIdx.x % kRedAtoms, sl = threadIdx.x / kRedAtoms;
+    const int i = blockIdx.x * kRedAtoms + al;
+    const bool valid = i < a.ntot;
+    double s[comps];
+    for (int k = 0; k < comps; ++k) s[k] = 0.0;
+    if (valid) {
+        const WinDesc W = a.wins[a.atom_win[i]];
+        const long long base = W.ppo + (i - W.a0);
+        for (int c = sl; c < W.chunks; c += kRedSlices) {
+            const long long o = base + (long long)c * W.n;
+            for (int k = 0; k < comps; ++k) s[k] += a.part[k * a.C + o];
+        }
     }
+    if (sl > 0)
+        for (int k = 0; k < comps; ++k) red[sl - 1][k][al] = s[k];
+    __syncthreads();
+    if (sl != 0 || !valid) return;
+    for (int t = 0; t < kRedSlices - 1; ++t)
+        for (int k = 0; k < comps; ++k) s[k] += red[t][k][al];
     const float4 x = a.xyzq[i];
     const double vt = (double)a.vtot[a.itype[i]];
     const double qv = -(double)x.w * vt;
@@ -616,10 +635,11 @@ hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t
 hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s)
 {
     if (a.ntot == 0) return hipSuccess;
+    const int nb = (a.ntot + kRedAtoms - 1) / kRedAtoms;
     if (split)
-        hipLaunchKernelGGL(k_force_reduce<true>, dim3(blocks_for(a.ntot)), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_force_reduce<true>, dim3(nb), dim3(kBlock), 0, s, a);
     else
-        hipLaunchKernelGGL(k_force_reduce<false>, dim3(blocks_for(a.ntot)), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(k_force_reduce<false>, dim3(nb), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -13,13 +13,15 @@ back-to-back launches of each stage) and keeps the fastest.
 
 Two candidate families:
   * target wave counts (the automatic per-window choice at several targets),
-    the only sensible knob for batches of uneven windows;
-  * explicit (splits, chunks) pairs for a single system: a x0.5/x1/x2 grid
-    around the tiling of the best wave target, plus the pair of the best split count and
-    the best chunk count found separately.  The field pass and its epilogue
-    depend only on the split count and the force pass and its reduction only
-    on the chunk count, so each is searched on its own stages, over counts
-    whose grid fills at least 85% of its last round of wave slots.
+    which share the parallelism among uneven windows by their work;
+  * explicit (splits, chunks) pairs: a x0.5/x1/x2 grid around the tiling of
+    the best wave target, plus the pair of the best split count and the best
+    chunk count found separately.  The field pass and its epilogue depend only
+    on the split count and the force pass and its reduction only on the chunk
+    count, so each is searched on its own stages: for a single system over
+    counts whose grid fills at least 85% of its last round of wave slots, for
+    a batch over x0.25..x2 of the largest window's counts (an explicit tiling
+    applies to every window, clamped to its size).
 The choice is deterministic given the timings; ties keep the earlier
 candidate.  Changing the tiling rebuilds the launch tables, so tune before
 capturing a graph.
@@ -84,8 +86,8 @@ def tune(g, reps: int = 3, waves: Sequence[int] = (8192, 16384, 24576, 32768, 49
                                                   98304, 131072),
          tilings: Optional[Sequence[Tuple[int, int]]] = None, log=None) -> TuneResult:
     """Pick the fastest launch geometry for GpuSystem `g`'s IS-SPA model and
-    leave it applied.  `tilings` defaults to a grid around the automatic
-    choice for a single system and to none for a batch."""
+    leave it applied.  `tilings` defaults to the explicit candidates described
+    above; pass () for wave targets only."""
     table = []
     for wv in waves:
         g.set_tiling(0, 0)
@@ -94,21 +96,27 @@ def tune(g, reps: int = 3, waves: Sequence[int] = (8192, 16384, 24576, 32768, 49
         table.append(("waves", (wv,), ms))
         if log:
             log(f"autotune waves={wv}: {ms:.4f} ms  tiling {g.tiling()}")
-    if tilings is None and g.windows is None:
-        # centre the explicit candidates on the best wave target's tiling
+    if tilings is None:
+        # centre the explicit candidates on the best wave target's tiling (for a
+        # batch: the largest window's; an explicit tiling applies to every
+        # window, clamped to its size)
         best_wv = min(table, key=lambda t: t[2])[1][0] if table else 32768
         g.set_tiling(0, 0)
         g.set_target_waves(best_wv)
         s0, c0 = g.tiling()
         grid = {(max(1, int(s0 * a)), max(1, int(c0 * b)))
                 for a in (0.5, 1.0, 2.0) for b in (0.5, 1.0, 2.0)}
-        n, P = g.system.n_atoms, g.system.n_points
+        if g.windows is None:
+            n, P = g.system.n_atoms, g.system.n_points
+            split_c = round_aligned(n, -(-P // 256), s0)
+            chunk_c = round_aligned(P, -(-n // 256), c0)
+        else:
+            split_c = sorted({max(1, s0 // 4), max(1, s0 // 2), s0, 2 * s0})
+            chunk_c = sorted({max(1, c0 // 4), max(1, c0 // 2), c0, 2 * c0})
         best = []
         for kind, cands, stages, pick in (
-                ("field-splits", round_aligned(n, -(-P // 256), s0), FIELD_STAGES,
-                 lambda v: (v, c0)),
-                ("force-chunks", round_aligned(P, -(-n // 256), c0), FORCE_STAGES,
-                 lambda v: (s0, v))):
+                ("field-splits", split_c, FIELD_STAGES, lambda v: (v, c0)),
+                ("force-chunks", chunk_c, FORCE_STAGES, lambda v: (s0, v))):
             stage_ms = []
             for v in cands:
                 g.set_tiling(*pick(v))
