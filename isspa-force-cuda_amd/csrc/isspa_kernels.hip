@@ -30,7 +30,7 @@ namespace isspa {
 
 constexpr int kBlock = 256;
 constexpr int kFieldTile = 128;
-constexpr int kForceTile = 64;
+constexpr int kForceTile = 128;
 // per-type tables up to this many floats (16 KB) are staged in LDS for the
 // in-sphere branches; 8 resident blocks per CU then hold 128 of its 160 KB
 constexpr int kTabLdsFloats = 4096;
@@ -156,7 +156,7 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
                                            float &ay, float &az, FieldRare &st)
 {
     const float4 *__restrict__ xq = a.xyzq;
-#pragma unroll 4
+#pragma unroll 2
     for (int j = j0; j < j1; ++j) {
         const float4 q = xq[j];  // wave-uniform
         float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
