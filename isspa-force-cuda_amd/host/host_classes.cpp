@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -94,7 +95,7 @@ static int read_rst7_into(const char *path, int nAtoms, float4 *dst, const char 
         for (int c = 0; c < 6 && (int)vals.size() < 3 * nAtoms; ++c) {
             if ((size_t)(c * 12) >= len) break;
             char tok[13] = {0};
-            strncpy(tok, line + c * 12, 12);
+            memcpy(tok, line + c * 12, std::min<size_t>(12, len - (size_t)(c * 12)));
             vals.push_back((float)atof(tok));
         }
     }
@@ -305,7 +306,7 @@ int isspa::read_isspa_prmtop(const char *path, int configMC)
     auto field = [](const char *l, int col, int w) {
         char tok[32] = {0};
         const size_t len = strlen(l);
-        if ((size_t)col < len) strncpy(tok, l + col, (size_t)w);
+        if ((size_t)col < len) memcpy(tok, l + col, std::min<size_t>({(size_t)w, len - col, 31}));
         return std::string(tok);
     };
     int n = -1, status = 0;
