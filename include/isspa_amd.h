@@ -252,6 +252,14 @@ int isspa_diff_forces(isspa_diff *m, const float *xyz4_d, const double *q_d,
 int isspa_diff_vjp(isspa_diff *m, const float *xyz4_d, const double *q_d,
                    const double *theta_d, const double *lam_d, double *gq_d, double *gtheta_d,
                    double *gatom_d, void *stream);
+/* the same without re-running the forward: uses the MC points, per-point
+ * records and S_e of the last isspa_diff_forces on this model, which must have
+ * been given the same xyz4_d, q_d and theta_d buffers (the caller guarantees
+ * their contents are unchanged) under the current tiling; ISSPA_ERR_STATE
+ * otherwise.  A plain isspa_diff_vjp in between invalidates that state. */
+int isspa_diff_vjp_after_forces(isspa_diff *m, const float *xyz4_d, const double *q_d,
+                                const double *theta_d, const double *lam_d, double *gq_d,
+                                double *gtheta_d, double *gatom_d, void *stream);
 
 /* ------------------------------------------------------------------------- */
 /* Leap-frog + Andersen: leapfrog_cuda (src/leapfrog_cuda.cu:73-86, kernel    */

@@ -26,6 +26,11 @@ struct isspa_diff {
     double *ppart = nullptr, *apart = nullptr;
     float4 *prec = nullptr;
     long long capF = 0, capC = 0;
+    // inputs and tiling generation of the last isspa_diff_forces, whose points,
+    // records and S_e isspa_diff_vjp_after_forces reuses
+    const void *fwd_x = nullptr, *fwd_q = nullptr, *fwd_theta = nullptr;
+    unsigned fwd_gen = 0;
+    bool fwd_valid = false;
 };
 
 static void diff_free(isspa_diff *m)
@@ -206,25 +211,34 @@ extern "C" int isspa_diff_forces(isspa_diff *m, const float *xyz4_d, const doubl
     if (!m || !xyz4_d || !q_d || !theta_d || !f_d)
         return fail(ISSPA_ERR_ARG, "isspa_diff_forces: null argument");
     hipStream_t s = S(stream);
+    m->fwd_valid = false;
     CHK(diff_buffers(m));
     CHK(diff_prepare(m, xyz4_d, q_d, theta_d, nullptr, s));
     CHK(diff_isspa_forward(m, q_d, f_d, s));
-    return nonbond_run(m->nb, reinterpret_cast<const float *>(m->xq), false, nullptr, f_d,
-                       nullptr, nullptr, s);
+    CHK(nonbond_run(m->nb, reinterpret_cast<const float *>(m->xq), false, nullptr, f_d, nullptr,
+                    nullptr, s));
+    m->fwd_x = xyz4_d;
+    m->fwd_q = q_d;
+    m->fwd_theta = theta_d;
+    m->fwd_gen = m->im->generation;
+    m->fwd_valid = true;
+    return ISSPA_OK;
 }
 
-extern "C" int isspa_diff_vjp(isspa_diff *m, const float *xyz4_d, const double *q_d,
-                              const double *theta_d, const double *lam_d, double *gq_d,
-                              double *gtheta_d, double *gatom_d, void *stream)
+// the adjoint; with run_forward false the points, records and S_e of the last
+// isspa_diff_forces on the same inputs are used as they are
+static int diff_vjp(isspa_diff *m, const float *xyz4_d, const double *q_d, const double *theta_d,
+                    const double *lam_d, double *gq_d, double *gtheta_d, double *gatom_d,
+                    bool run_forward, hipStream_t s)
 {
-    if (!m || !xyz4_d || !q_d || !theta_d || !lam_d || !gq_d || !gtheta_d)
-        return fail(ISSPA_ERR_ARG, "isspa_diff_vjp: null argument");
-    hipStream_t s = S(stream);
     isspa_model *im = m->im;
     nonbond_model *nb = m->nb;
     CHK(diff_buffers(m));
     CHK(diff_prepare(m, xyz4_d, q_d, theta_d, lam_d, s));
-    CHK(diff_isspa_forward(m, q_d, m->ftmp, s));
+    if (run_forward) {
+        m->fwd_valid = false;  // ftmp, not the caller's forces, holds this forward
+        CHK(diff_isspa_forward(m, q_d, m->ftmp, s));
+    }
     VjpArgs va{};
     va.wins = im->wins;
     va.atom_win = im->atom_win;
@@ -292,4 +306,28 @@ extern "C" int isspa_diff_vjp(isspa_diff *m, const float *xyz4_d, const double *
     HIPCHK(launch_vjp_finish(fa, s));
     HIPCHK(launch_type_reduce(fa.gatom, im->itype, m->n, m->T, gtheta_d, s));
     return ISSPA_OK;
+}
+
+extern "C" int isspa_diff_vjp(isspa_diff *m, const float *xyz4_d, const double *q_d,
+                              const double *theta_d, const double *lam_d, double *gq_d,
+                              double *gtheta_d, double *gatom_d, void *stream)
+{
+    if (!m || !xyz4_d || !q_d || !theta_d || !lam_d || !gq_d || !gtheta_d)
+        return fail(ISSPA_ERR_ARG, "isspa_diff_vjp: null argument");
+    return diff_vjp(m, xyz4_d, q_d, theta_d, lam_d, gq_d, gtheta_d, gatom_d, true, S(stream));
+}
+
+extern "C" int isspa_diff_vjp_after_forces(isspa_diff *m, const float *xyz4_d, const double *q_d,
+                                           const double *theta_d, const double *lam_d,
+                                           double *gq_d, double *gtheta_d, double *gatom_d,
+                                           void *stream)
+{
+    if (!m || !xyz4_d || !q_d || !theta_d || !lam_d || !gq_d || !gtheta_d)
+        return fail(ISSPA_ERR_ARG, "isspa_diff_vjp_after_forces: null argument");
+    if (!m->fwd_valid || m->fwd_x != xyz4_d || m->fwd_q != q_d || m->fwd_theta != theta_d ||
+        m->fwd_gen != m->im->generation)
+        return fail(ISSPA_ERR_STATE,
+                    "isspa_diff_vjp_after_forces: the last isspa_diff_forces on this model did "
+                    "not use these buffers under the current tiling");
+    return diff_vjp(m, xyz4_d, q_d, theta_d, lam_d, gq_d, gtheta_d, gatom_d, false, S(stream));
 }

@@ -111,6 +111,7 @@ _SIGS = {
     "isspa_diff_set_tiling": ([vp, i32, i32], i32),
     "isspa_diff_forces": ([vp, vp, vp, vp, vp, vp], i32),
     "isspa_diff_vjp": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+    "isspa_diff_vjp_after_forces": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -137,6 +137,11 @@ class DiffForces:
         self.h = h
         self.x = torch.from_numpy(np.ascontiguousarray(system.xyzq, np.float32)).to(self.dev)
         self._capi = capi
+        # the (q, theta) buffers of the last forces() call, kept alive so the
+        # adjoint can reuse that forward (isspa_diff_vjp_after_forces), and a
+        # token replaced by every forces() and plain vjp() call
+        self._fwd = None
+        self.state = object()
 
     def _stream(self):
         import torch
@@ -153,20 +158,33 @@ class DiffForces:
         self._capi.check(self._capi.lib.isspa_diff_forces(
             self.h, self.x.data_ptr(), q.data_ptr(), theta.data_ptr(), f.data_ptr(),
             self._stream()), "isspa_diff_forces")
+        self._fwd = (q, theta)
+        self.state = object()
         return f
 
-    def vjp(self, q, theta, lam, per_atom: bool = False):
+    def vjp(self, q, theta, lam, per_atom: bool = False, after_forces: bool = False):
+        """dL/dq and dL/dtheta for lam = dL/dF.  With after_forces the last
+        forces() call's forward is reused (q and theta are then those of that
+        call; the C ABI refuses if its state is gone)."""
         import torch
-        q = q.detach().to(self.dev, torch.float64).contiguous()
-        theta = theta.detach().to(self.dev, torch.float64).contiguous()
+        if after_forces:
+            if self._fwd is None:
+                raise RuntimeError("vjp(after_forces=True) needs a preceding forces() call")
+            q, theta = self._fwd
+            fn, name = self._capi.lib.isspa_diff_vjp_after_forces, "isspa_diff_vjp_after_forces"
+        else:
+            q = q.detach().to(self.dev, torch.float64).contiguous()
+            theta = theta.detach().to(self.dev, torch.float64).contiguous()
+            fn, name = self._capi.lib.isspa_diff_vjp, "isspa_diff_vjp"
+            self._fwd = None
+            self.state = object()
         lam = lam.detach().to(self.dev, torch.float64).contiguous()
         gq = torch.empty(self.n, dtype=torch.float64, device=self.dev)
         gt = torch.empty((self.T, 5), dtype=torch.float64, device=self.dev)
         ga = torch.empty((self.n, 5), dtype=torch.float64, device=self.dev) if per_atom else None
-        self._capi.check(self._capi.lib.isspa_diff_vjp(
-            self.h, self.x.data_ptr(), q.data_ptr(), theta.data_ptr(), lam.data_ptr(),
-            gq.data_ptr(), gt.data_ptr(), ga.data_ptr() if per_atom else None, self._stream()),
-            "isspa_diff_vjp")
+        self._capi.check(fn(self.h, self.x.data_ptr(), q.data_ptr(), theta.data_ptr(),
+                            lam.data_ptr(), gq.data_ptr(), gt.data_ptr(),
+                            ga.data_ptr() if per_atom else None, self._stream()), name)
         return (gq, gt, ga) if per_atom else (gq, gt)
 
     def close(self):
@@ -186,18 +204,24 @@ def _autograd_function():
 
     class IsspaForces(torch.autograd.Function):
         """F(theta, q) of a DiffForces model; backward is the hand-written
-        adjoint (isspa_diff_vjp), not autograd through the kernels."""
+        adjoint (isspa_diff_vjp), not autograd through the kernels.  When no
+        other forces() or vjp() call ran on the model since this forward, the
+        backward reuses the forward's points and records
+        (isspa_diff_vjp_after_forces) instead of recomputing them."""
 
         @staticmethod
         def forward(ctx, theta, q, model):
             ctx.model = model
             ctx.save_for_backward(theta, q)
-            return model.forces(q, theta)
+            f = model.forces(q, theta)
+            ctx.state = model.state
+            return f
 
         @staticmethod
         def backward(ctx, gf):
-            theta, q = ctx.saved_tensors
-            gq, gt = ctx.model.vjp(q, theta, gf)
+            theta, q = ctx.saved_tensors  # raises if modified in place since forward
+            reuse = ctx.model.state is ctx.state
+            gq, gt = ctx.model.vjp(q, theta, gf, after_forces=reuse)
             return gt.to(theta.dtype), gq.to(q.dtype), None
 
     return IsspaForces

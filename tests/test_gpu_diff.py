@@ -133,3 +133,33 @@ def test_autograd_function_and_force_matching_step(small):
     print(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
     assert losses[-1] < 0.9 * losses[0]
     m.close()
+
+
+def test_vjp_after_forces_reuses_the_forward(small):
+    """isspa_diff_vjp_after_forces (the adjoint without re-running the
+    forward) equals isspa_diff_vjp bit for bit, and refuses when the forward
+    state it would reuse is gone: before any forces(), after a plain vjp(),
+    and after a tiling change."""
+    s = small
+    prm = DiffParams.from_system(s)
+    m = DiffForces(s, prm, step=2)
+    th, q = _t(prm.theta), _t(prm.q)
+    lam = _t(np.random.default_rng(5).normal(size=(s.n_atoms, 3)))
+    with pytest.raises(RuntimeError):
+        m.vjp(q, th, lam, after_forces=True)
+    m.forces(q, th)
+    gq_r, gt_r, ga_r = m.vjp(q, th, lam, per_atom=True, after_forces=True)
+    gq_p, gt_p, ga_p = m.vjp(q, th, lam, per_atom=True)
+    torch.cuda.synchronize()
+    assert torch.equal(gq_r, gq_p) and torch.equal(gt_r, gt_p) and torch.equal(ga_r, ga_p)
+    with pytest.raises(RuntimeError):  # the plain vjp replaced the forward state
+        m.vjp(q, th, lam, after_forces=True)
+    m.forces(q, th)
+    m.set_tiling(2, 3)
+    with pytest.raises(RuntimeError):  # launch tables rebuilt since that forward
+        m._capi.check(m._capi.lib.isspa_diff_vjp_after_forces(
+            m.h, m.x.data_ptr(), m._fwd[0].data_ptr(), m._fwd[1].data_ptr(), lam.data_ptr(),
+            torch.empty(s.n_atoms, dtype=torch.float64, device="cuda:0").data_ptr(),
+            torch.empty((prm.n_types, 5), dtype=torch.float64, device="cuda:0").data_ptr(),
+            None, m._stream()), "isspa_diff_vjp_after_forces")
+    m.close()
