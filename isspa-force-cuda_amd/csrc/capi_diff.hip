@@ -254,6 +254,7 @@ static int diff_vjp(isspa_diff *m, const float *xyz4_d, const double *q_d, const
     va.itype = im->itype;
     va.rmax2 = im->rmax2;
     va.dtypes = m->dtypes;
+    va.ntypes = m->T;
     va.pbc_flag = im->pbc_flag;
     va.g_out = im->g_out;
     va.E_out = im->E_out;

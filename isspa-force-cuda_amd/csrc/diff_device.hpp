@@ -63,6 +63,26 @@ __device__ __forceinline__ float diff_E(const DiffType &T, float d)
     return T.c * expf(-0.5f * u * u) * diff_switch(T, d).S;
 }
 
+// the four constants diff_F needs, held in registers by the force pass
+struct DiffFParams {
+    float sts, ets, rlo, idel;
+};
+
+__device__ __forceinline__ DiffFParams diff_f_params(const DiffType &T)
+{
+    return DiffFParams{T.sts, T.ets, T.rlo, T.idel};
+}
+
+__device__ __forceinline__ float diff_F(const DiffFParams &T, float d, float rinv)
+{
+    const float z = (d - T.rlo) * T.idel + 1.0f;
+    const float S = z <= 0.0f ? 0.0f : (z >= 1.0f ? 1.0f : z * z * (3.0f - 2.0f * z));
+    const float sr = T.sts * rinv;
+    const float sr2 = sr * sr;
+    const float sr6 = sr2 * sr2 * sr2;
+    return 24.0f * T.ets * rinv * (2.0f * sr6 * sr6 - sr6) * S;
+}
+
 __device__ __forceinline__ float diff_F(const DiffType &T, float d, float rinv)
 {
     const float sr = T.sts * rinv;

@@ -118,6 +118,25 @@ __global__ __launch_bounds__(kB) void k_diff_reduce(DiffReduceArgs a)
 // ---------------------------------------------------------------------------
 // adjoint, point side: lanes own MC points, receivers stream (wave-uniform)
 // ---------------------------------------------------------------------------
+// per-type records of the analytic model in LDS for the in-sphere branches
+// when they fit (at most kDtLdsFloats floats); the whole block calls this
+constexpr int kDtLdsFloats = 4096;
+__host__ __device__ inline int dtype_floats(int ntypes)
+{
+    return ntypes * (int)(sizeof(DiffType) / sizeof(float));
+}
+
+__device__ __forceinline__ const DiffType *stage_dtypes(const VjpArgs &a)
+{
+    extern __shared__ float lds_dt[];
+    const int nf = dtype_floats(a.ntypes);
+    if (nf > kDtLdsFloats) return a.dtypes;
+    const float *src = reinterpret_cast<const float *>(a.dtypes);
+    for (int t = threadIdx.x; t < nf; t += kB) lds_dt[t] = src[t];
+    __syncthreads();
+    return reinterpret_cast<const DiffType *>(lds_dt);
+}
+
 struct PointBar {
     float hx, hy, hz;   // E-hat bar
     float a1, a2, a3;   // A1..A3 bar
@@ -126,10 +145,10 @@ struct PointBar {
 };
 
 template <bool PBC>
-__device__ __forceinline__ void vjp_points_tile(const VjpArgs &a, int j0, int j1, float L,
-                                                float h, float px, float py, float pz, float Ex,
-                                                float Ey, float Ez, float A1, float A2, float A3,
-                                                PointBar &b)
+__device__ __forceinline__ void vjp_points_tile(const VjpArgs &a, const DiffType *dtypes, int j0,
+                                                int j1, float L, float h, float px, float py,
+                                                float pz, float Ex, float Ey, float Ez, float A1,
+                                                float A2, float A3, PointBar &b)
 {
     const float4 *__restrict__ ar = a.arec;
     for (int j = j0; j < j1; ++j) {
@@ -177,7 +196,7 @@ __device__ __forceinline__ void vjp_points_tile(const VjpArgs &a, int j0, int j1
         } else {
             const float lp = __builtin_fmaf(r2.z, rz, __builtin_fmaf(r2.y, ry, r2.x * rx));
             const int ti = __float_as_int(r1.w);
-            b.w += diff_F(a.dtypes[ti], d2 * rinv, rinv) * lp * rinv;
+            b.w += diff_F(dtypes[ti], d2 * rinv, rinv) * lp * rinv;
         }
     }
 }
@@ -198,15 +217,16 @@ __global__ __launch_bounds__(kB) void k_vjp_points(VjpArgs a)
     const float4 q3 = a.rec4[4 * gp + 3];  // A2, 3 A2, A3, w
     const float A1 = a.aux4[gp].w;
     const bool pbc = a.pbc_flag[bt.x] != 0;
+    const DiffType *dtypes = stage_dtypes(a);
     double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int jt = j0; jt < j1; jt += kTile) {
         const int je = min(j1, jt + kTile);
         PointBar b{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (pbc)
-            vjp_points_tile<true>(a, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, A1,
+            vjp_points_tile<true>(a, dtypes, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, A1,
                                   q3.x, q3.z, b);
         else
-            vjp_points_tile<false>(a, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, A1,
+            vjp_points_tile<false>(a, dtypes, jt, je, W.L, W.h, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, A1,
                                    q3.x, q3.z, b);
         acc[0] += b.hx; acc[1] += b.hy; acc[2] += b.hz;
         acc[3] += b.a1; acc[4] += b.a2; acc[5] += b.a3;
@@ -303,14 +323,13 @@ __global__ __launch_bounds__(kB) void k_vjp_epilogue(VjpArgs a)
 // adjoint, atom side: lanes own atoms, point records stream (wave-uniform)
 // ---------------------------------------------------------------------------
 template <bool PBC>
-__device__ __forceinline__ void vjp_atoms_tile(const VjpArgs &a, long long m0, long long m1,
-                                               float L, float h, float4 xj, float R2j, int tj,
-                                               float lpx, float lpy, float lpz, float &qb,
-                                               double tb[5])
+__device__ __forceinline__ void vjp_atoms_tile(const VjpArgs &a, const float4 *__restrict__ pr,
+                                               const DiffType &Tj, int m0, int m1, float L,
+                                               float h, float4 xj, float R2j, float lpx,
+                                               float lpy, float lpz, float &qb, double tb[5])
 {
-    const float4 *__restrict__ pr = a.prec;
     const float e0 = a.e0;
-    for (long long m = m0; m < m1; ++m) {
+    for (int m = m0; m < m1; ++m) {
         const float4 p = pr[4 * m + 0];  // wave-uniform
         const float4 eb = pr[4 * m + 1];
         const float4 e0b = pr[4 * m + 2];
@@ -331,7 +350,7 @@ __device__ __forceinline__ void vjp_atoms_tile(const VjpArgs &a, long long m0, l
         if (in_own || in_rec) {
             const float d = d2 * rinv;
             float v[3], dv[15];
-            diff_all(a.dtypes[tj], d, rinv, v, dv);
+            diff_all(Tj, d, rinv, v, dv);
             if (in_own) {
                 const float ce = er * rinv;
                 const float cg = (eb.w != 0.0f && v[0] != 0.0f) ? eb.w / v[0] : 0.0f;
@@ -353,8 +372,9 @@ __global__ __launch_bounds__(kB) void k_vjp_atoms(VjpArgs a)
     const int il = bt.y * kB + threadIdx.x;
     const int chunk = bt.z;
     const long long pbase = (long long)W.a0 * a.nmc;
-    const long long m0 = pbase + (long long)chunk * W.ppc;
-    const long long m1 = min(pbase + (long long)W.n * a.nmc, m0 + W.ppc);
+    const float4 *__restrict__ pr = a.prec + 4 * pbase;  // this window's point bars
+    const int m0 = chunk * W.ppc;                        // window-local point range
+    const int m1 = min(W.n * a.nmc, m0 + W.ppc);
     const bool valid = il < W.n;
     const int j = W.a0 + (valid ? il : 0);
     const float4 xj = a.xq[j];
@@ -362,14 +382,15 @@ __global__ __launch_bounds__(kB) void k_vjp_atoms(VjpArgs a)
     const float R2j = valid ? a.rmax2[tj] : -1.0f;
     const float4 lp = a.arec[4 * (size_t)j + 2];  // Lambda' = -vtot lambda
     const bool pbc = a.pbc_flag[bt.x] != 0;
+    const DiffType &Tj = stage_dtypes(a)[tj];
     double qacc = 0.0, tb[5] = {0, 0, 0, 0, 0};
-    for (long long mt = m0; mt < m1; mt += kTile) {
-        const long long me = min(m1, mt + kTile);
+    for (int mt = m0; mt < m1; mt += kTile) {
+        const int me = min(m1, mt + kTile);
         float qb = 0.0f;
         if (pbc)
-            vjp_atoms_tile<true>(a, mt, me, W.L, W.h, xj, R2j, tj, lp.x, lp.y, lp.z, qb, tb);
+            vjp_atoms_tile<true>(a, pr, Tj, mt, me, W.L, W.h, xj, R2j, lp.x, lp.y, lp.z, qb, tb);
         else
-            vjp_atoms_tile<false>(a, mt, me, W.L, W.h, xj, R2j, tj, lp.x, lp.y, lp.z, qb, tb);
+            vjp_atoms_tile<false>(a, pr, Tj, mt, me, W.L, W.h, xj, R2j, lp.x, lp.y, lp.z, qb, tb);
         qacc += (double)qb;
     }
     if (!valid) return;
@@ -540,7 +561,9 @@ hipError_t launch_diff_reduce(const DiffReduceArgs &a, hipStream_t s)
 hipError_t launch_vjp_points(const VjpArgs &a, int nblocks, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vjp_points, dim3(nblocks), dim3(kB), 0, s, a);
+    const int nf = dtype_floats(a.ntypes);
+    const size_t lds = nf <= kDtLdsFloats ? (size_t)nf * sizeof(float) : 0;
+    hipLaunchKernelGGL(k_vjp_points, dim3(nblocks), dim3(kB), lds, s, a);
     return hipGetLastError();
 }
 
@@ -554,7 +577,9 @@ hipError_t launch_vjp_epilogue(const VjpArgs &a, hipStream_t s)
 hipError_t launch_vjp_atoms(const VjpArgs &a, int nblocks, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vjp_atoms, dim3(nblocks), dim3(kB), 0, s, a);
+    const int nf = dtype_floats(a.ntypes);
+    const size_t lds = nf <= kDtLdsFloats ? (size_t)nf * sizeof(float) : 0;
+    hipLaunchKernelGGL(k_vjp_atoms, dim3(nblocks), dim3(kB), lds, s, a);
     return hipGetLastError();
 }
 

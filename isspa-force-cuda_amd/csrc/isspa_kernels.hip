@@ -40,6 +40,11 @@ __host__ __device__ inline int field_tab_floats(const TableView &t)
     return t.ntypes * (t.ld_g + t.ld_e);
 }
 __host__ __device__ inline int force_tab_floats(const TableView &t) { return t.ntypes * t.ld_f; }
+// per-type records of the analytic model (one DiffType per type)
+__host__ __device__ inline int diff_tab_floats(const TableView &t)
+{
+    return t.ntypes * (int)(sizeof(DiffType) / sizeof(float));
+}
 
 // copy n floats into LDS with the whole block (n is block-uniform)
 __device__ __forceinline__ void stage_lds(float *dst, const float *__restrict__ src, int n)
@@ -151,7 +156,8 @@ __device__ __forceinline__ void field_in_sphere_an(const DiffType &T, float d2, 
 
 template <bool PBC, bool AN>
 __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab,
-                                           const float *Etab, int j0, int j1, float L, float h,
+                                           const float *Etab, const DiffType *dtypes, int j0,
+                                           int j1, float L, float h,
                                            float px, float py, float pz, float R2, float &ax,
                                            float &ay, float &az, FieldRare &st)
 {
@@ -175,7 +181,7 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
         az = __builtin_fmaf(so, rz, az);
         if (in) {
             if (AN)
-                field_in_sphere_an(a.dtypes[a.itype[j]], d2, inv, s, rx, ry, rz, a.e0, st);
+                field_in_sphere_an(dtypes[a.itype[j]], d2, inv, s, rx, ry, rz, a.e0, st);
             else
                 field_in_sphere(a.tab, Gtab, Etab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0,
                                 st);
@@ -200,6 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     // g and E tables for the in-sphere branch: LDS when they fit
     extern __shared__ float lds_tab[];
     const float *Gtab = a.tab.g, *Etab = a.tab.e;
+    const DiffType *dtypes = a.dtypes;
     if (!AN && field_tab_floats(a.tab) <= kTabLdsFloats) {
         const int ng = a.tab.ntypes * a.tab.ld_g;
         stage_lds(lds_tab, a.tab.g, ng);
@@ -208,6 +215,11 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
         Gtab = lds_tab;
         Etab = lds_tab + ng;
     }
+    if (AN && diff_tab_floats(a.tab) <= kTabLdsFloats) {
+        stage_lds(lds_tab, reinterpret_cast<const float *>(a.dtypes), diff_tab_floats(a.tab));
+        __syncthreads();
+        dtypes = reinterpret_cast<const DiffType *>(lds_tab);
+    }
     double sx = 0.0, sy = 0.0, sz = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
     FieldRare st{1.0f, 0, 0.0f, 0.0f, 0.0f};
     for (int jt = j0; jt < j1; jt += kFieldTile) {
@@ -215,11 +227,11 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         st.ex = st.ey = st.ez = 0.0f;
         if (pbc)
-            field_tile<true, AN>(a, Gtab, Etab, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay, az,
-                                 st);
+            field_tile<true, AN>(a, Gtab, Etab, dtypes, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax,
+                                 ay, az, st);
         else
-            field_tile<false, AN>(a, Gtab, Etab, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax, ay,
-                                  az, st);
+            field_tile<false, AN>(a, Gtab, Etab, dtypes, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax,
+                                  ay, az, st);
         sx += (double)ax;
         sy += (double)ay;
         sz += (double)az;
@@ -372,8 +384,8 @@ template <bool PBC, bool SPLIT, bool AN>
 __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__restrict__ rec,
                                            const float4 *__restrict__ aux, int m0, int m1,
                                            float L, float h, float xi, float yi, float zi,
-                                           float R2, const float *Ftab, int ti, TileSums &ts,
-                                           ForceAcc &acc)
+                                           float R2, const float *Ftab,
+                                           const DiffFParams &fp, TileSums &ts, ForceAcc &acc)
 {
     float &sx = ts.sx, &sy = ts.sy, &sz = ts.sz;
     // 32-bit point indices within the window keep the loop test on the SALU
@@ -439,7 +451,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
             const float d = d2 * rinv;
             float fs = 0.0f;
             if (AN) {
-                fs = diff_F(a.dtypes[ti], d, rinv) * w;
+                fs = diff_F(fp, d, rinv) * w;
             } else {
                 const int b = (int)((d - a.tab.f_r0) * a.tab.f_inv + 0.5f);
                 if (b >= 0 && b < a.tab.n_f) {
@@ -479,9 +491,12 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const float4 xi = a.xyzq[i];
     const int ti = a.itype[i];
     const float R2 = valid ? a.rmax2[ti] : -1.0f;
-    // the receiver type's F table for the in-sphere branch: LDS when it fits
+    // the receiver type's F table for the in-sphere branch: LDS when it fits;
+    // the analytic model keeps its four F constants in registers
     extern __shared__ float lds_tab[];
     const float *Ftab = nullptr;
+    DiffFParams fp{0.0f, 0.0f, 0.0f, 0.0f};
+    if (AN) fp = diff_f_params(a.dtypes[ti]);
     if (!AN) {
         const float *ftab = a.tab.f;
         if (force_tab_floats(a.tab) <= kTabLdsFloats) {
@@ -498,10 +513,10 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
         TileSums ts{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
         if (pbc)
             force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
-                                        ti, ts, acc);
+                                        fp, ts, acc);
         else
             force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
-                                         ti, ts, acc);
+                                         fp, ts, acc);
         acc.ex += (double)ts.sx;
         acc.ey += (double)ts.sy;
         acc.ez += (double)ts.sz;
@@ -597,7 +612,9 @@ hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
     if (a.dtypes) {
-        hipLaunchKernelGGL(k_field<true>, dim3(nblocks), dim3(kBlock), 0, s, a);
+        const int nt = diff_tab_floats(a.tab);
+        const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
+        hipLaunchKernelGGL(k_field<true>, dim3(nblocks), dim3(kBlock), lds, s, a);
     } else {
         const int nt = field_tab_floats(a.tab);
         const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;

@@ -220,6 +220,7 @@ struct VjpArgs {
     const int *itype;
     const float *rmax2;
     const DiffType *dtypes;
+    int ntypes;
     const int *pbc_flag;
     const double *g_out, *E_out;
     const int *cnt_out;
