@@ -115,6 +115,7 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
     log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per IS-SPA pass")
     K = max(1, int(per_graph))
     steps = max(K, steps - steps % K)
+    run_plain_loop = plain_rounds > 0
     plain_rounds = max(K, plain_rounds - plain_rounds % K)
     g.set_step(0)
     g.capture(K)
@@ -129,6 +130,15 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
         t_b = _time_replays(run_batch, steps // K)
     if not np.all(np.isfinite(g.x.cpu().numpy())):
         raise RuntimeError("batched run produced non-finite positions")
+    if not run_plain_loop:
+        ms_b = 1e3 * t_b / steps
+        log(f"batched: {ms_b:.4f} ms per step of all {W} windows "
+            f"({W * 0.1728 * steps / t_b:.1f} window-ns/day); plain loop not run")
+        g.close()
+        return {"windows": W, "atoms": merged.n_atoms, "steps": steps, "steps_per_graph": K,
+                "ms_per_step": round(ms_b, 5),
+                "window_ns_per_day": round(W * 0.1728 * steps / t_b, 2),
+                "autotune": {"kind": tr.kind, "value": list(tr.value), "pass_ms": round(tr.ms, 5)}}
     plain = []
     for w, s in enumerate(windows):
         p = GpuSystem(s, "cuda:0", system_id=w)
@@ -185,7 +195,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched comparison and print its JSON object")
     ap.add_argument("--batch-steps", type=int, default=2000)
-    ap.add_argument("--plain-rounds", type=int, default=100)
+    ap.add_argument("--plain-rounds", type=int, default=100,
+                    help="rounds of the per-window loop in the batched comparison (0: skip it)")
     ap.add_argument("--steps-per-graph", type=int, default=1,
                     help="steps captured per graph in the batched comparison (both sides)")
     ap.add_argument("--tiling", default="", help="explicit field splits,force chunks (no tuning)")
