@@ -16,6 +16,35 @@ namespace isspa {
 constexpr int kWave = 64;
 
 // ---------------------------------------------------------------------------
+// Coefficients of the Langevin-moment series below x = 5:
+//   i_l(x) / x^l = sum_k T_k,  T_0 = 1/(2l+1)!!,  T_(k+1) = T_k h / ((k+1)(2l+2k+3)),
+// h = x^2/2.  The per-term divisors are compile-time constants, so the FP64
+// loops multiply by their reciprocals instead of dividing.
+// ---------------------------------------------------------------------------
+constexpr int kLgTerms = 30;
+struct LangevinCoef {
+    double first[4];           // 1 / (2l+1)!!
+    double step[4][kLgTerms];  // 1 / ((k+1)(2l+2k+3))
+    double inv[4][kLgTerms];   // 1 / (2l+2k+3)  (derivative of the series)
+};
+constexpr LangevinCoef make_langevin_coef()
+{
+    LangevinCoef c{};
+    double f = 1.0;
+    for (int l = 0; l < 4; ++l) {
+        if (l > 0) f /= (double)(2 * l + 1);
+        c.first[l] = f;
+        for (int k = 0; k < kLgTerms; ++k) {
+            const double d = (double)(2 * l + 2 * k + 3);
+            c.step[l][k] = 1.0 / ((double)(k + 1) * d);
+            c.inv[l][k] = 1.0 / d;
+        }
+    }
+    return c;
+}
+__constant__ constexpr LangevinCoef kLangevin = make_langevin_coef();
+
+// ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Counter layout:
 //   ctr = { index, (stream << 24) | draw, step, system }, key = { seed lo, hi }
 // ---------------------------------------------------------------------------

@@ -245,16 +245,14 @@ __device__ void moments_derivs(double x, double m[3], double dm[3])
     if (x < 5.0) {
         const double h = 0.5 * x * x;
         double s[4], ds[4];
-        double first = 1.0;
         for (int l = 0; l < 4; ++l) {
-            if (l > 0) first /= (double)(2 * l + 1);
-            // terms T_0 = first, T_(k+1) = T_k h / ((k+1)(2l+2k+3)), so
-            // d T_(k+1) / dh = T_k / (2l+2k+3)
-            double t = first, sum = first, dsum = 0.0;
-            for (int k = 0; k < 30; ++k) {
-                const double c = (double)(2 * l + 2 * k + 3);
-                dsum += t / c;
-                t *= h / ((double)(k + 1) * c);
+            // terms T_0 = 1/(2l+1)!!, T_(k+1) = T_k h / ((k+1)(2l+2k+3)), so
+            // d T_(k+1) / dh = T_k / (2l+2k+3); stop once a term is below
+            // 1e-17 of the sum
+            double t = kLangevin.first[l], sum = t, dsum = 0.0;
+            for (int k = 0; k < kLgTerms && t > 1e-17 * sum; ++k) {
+                dsum += t * kLangevin.inv[l][k];
+                t *= h * kLangevin.step[l][k];
                 sum += t;
             }
             s[l] = sum;

@@ -263,13 +263,12 @@ __device__ __forceinline__ void langevin_moments(double x, double &m1, double &m
     if (x < 5.0) {
         const double h = 0.5 * x * x;
         double s[4];
-        double first = 1.0;
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
-            if (l > 0) first /= (double)(2 * l + 1);
-            double t = first, sum = first;
-            for (int k = 0; k < 30; ++k) {
-                t *= h / ((double)(k + 1) * (double)(2 * l + 2 * k + 3));
+            // terms fall below 1e-17 of the sum well before k = 30 for small x
+            double t = kLangevin.first[l], sum = t;
+            for (int k = 0; k < kLgTerms && t > 1e-17 * sum; ++k) {
+                t *= h * kLangevin.step[l][k];
                 sum += t;
             }
             s[l] = sum;

@@ -351,19 +351,41 @@ hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+// A block owns 32 atoms and 8 slot slices: slice t sums slots t, t + 8, ...
+// of its atom in order, then slice 0 adds slices 1..7 in order (a fixed order
+// independent of scheduling, with 8 times the waves of one lane per atom).
+constexpr int kGatherAtoms = 32;
+constexpr int kGatherSlices = kBlk / kGatherAtoms;
+
 __global__ __launch_bounds__(kBlk) void k_bonded_gather(const float4 *__restrict__ contrib,
                                                         const int *__restrict__ ptr,
                                                         const int *__restrict__ idx, int n,
                                                         float4 *f)
 {
-    const int i = blockIdx.x * kBlk + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double red[kGatherSlices - 1][3][kGatherAtoms];
+    const int al = threadIdx.x % kGatherAtoms, sl = threadIdx.x / kGatherAtoms;
+    const int i = blockIdx.x * kGatherAtoms + al;
+    const bool valid = i < n;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int e = ptr[i]; e < ptr[i + 1]; ++e) {
-        const float4 c = contrib[idx[e]];
-        sx += c.x;
-        sy += c.y;
-        sz += c.z;
+    if (valid) {
+        for (int e = ptr[i] + sl; e < ptr[i + 1]; e += kGatherSlices) {
+            const float4 c = contrib[idx[e]];
+            sx += c.x;
+            sy += c.y;
+            sz += c.z;
+        }
+    }
+    if (sl > 0) {
+        red[sl - 1][0][al] = sx;
+        red[sl - 1][1][al] = sy;
+        red[sl - 1][2][al] = sz;
+    }
+    __syncthreads();
+    if (sl != 0 || !valid) return;
+    for (int t = 0; t < kGatherSlices - 1; ++t) {
+        sx += red[t][0][al];
+        sy += red[t][1][al];
+        sz += red[t][2][al];
     }
     float4 v = f[i];
     v.x += (float)sx;
@@ -376,8 +398,8 @@ hipError_t launch_bonded_gather(const float4 *contrib, const int *ptr, const int
                                 float4 *f, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bonded_gather, dim3(nblocks(n)), dim3(kBlk), 0, s, contrib, ptr, idx, n,
-                       f);
+    hipLaunchKernelGGL(k_bonded_gather, dim3((n + kGatherAtoms - 1) / kGatherAtoms), dim3(kBlk), 0,
+                       s, contrib, ptr, idx, n, f);
     return hipGetLastError();
 }
 
