@@ -77,6 +77,8 @@ static WinDesc base_desc(const Windows &L, int w)
 // windows of a batch need pieces this fine to fill the device.
 constexpr long long kMinSplitAtoms = 8;
 constexpr long long kMinChunkPoints = 16;
+// and an all-pairs partner chunk at least this many partners
+constexpr long long kMinNonbondPartners = 16;
 
 static int auto_pieces(long long lanes, long long work, long long min_work, double target)
 {
@@ -553,7 +555,7 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
         WinDesc d = base_desc(Ly, w);
         const long long nw = Ly.n[w];
         const double tgt = std::max(1.0, kWaveSlots * ((double)nw * nw / work_tot));
-        const int c = auto_pieces(nw, nw, 64, tgt);
+        const int c = auto_pieces(nw, nw, kMinNonbondPartners, tgt);
         d.apc = (int)((nw + c - 1) / c);
         d.nbc = (int)((nw + d.apc - 1) / d.apc);
         d.npo = B;
