@@ -159,3 +159,14 @@ def test_abi_rejects_bad_arguments_without_gpu():
     rc = capi.lib.us_model_create(10, 1, zi.ctypes.data, z.ctypes.data, z.ctypes.data, 1.0, 10.0,
                                   C.byref(h))
     assert rc == 1  # one group empty
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus 2 exits with a message and status 2
+    before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
