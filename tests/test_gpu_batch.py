@@ -157,3 +157,65 @@ def test_batch_graph_replay_matches_eager(merged):
         assert np.array_equal(eager.v.cpu().numpy(), graph.v.cpu().numpy()), f"replay {r}"
     eager.close()
     graph.close()
+
+
+def _mixed_box_windows():
+    """Four windows whose boxes make the minimum image fire in two of them:
+    a 22-atom chain in an 18 A box, 10 chains (220 atoms) in a 40 A box, and
+    a two-atom LJ pair and a chain in 200 A boxes."""
+    import copy
+    from isspa_amd import synth
+    from isspa_amd.system import UmbrellaRestraint
+    out = []
+    a = synth.make_chain()
+    a.lbox = 18.0
+    a.xyzq[:, :3] = (a.xyzq[:, :3] - a.xyzq[:, :3].mean(0) + 9.0).astype(np.float32)
+    b = synth.make_lattice(10, 40.0, 3, 40.0, seed=31)
+    c = synth.two_atom_lj(sep=5.0)
+    d = synth.make_chain()
+    d.xyzq[:, :3] += np.float32(100.0)
+    for s, (g1, g2) in zip((a, b, c, d), (([0], [21]), ([0], [22]), ([0], [1]), ([0], [21]))):
+        s.us = UmbrellaRestraint.from_groups(g1, g2, s.true_mass, 10.0, 6.0)
+        s.n_mc, s.temperature_k, s.pnu, s.seed = 8, 298.0, 0.01, 777
+        out.append(s)
+    return out
+
+
+def test_batch_mixed_boxes_minimum_image():
+    """Per-window boxes in one batch: windows whose minimum image fires sit
+    next to windows whose does not.  IS-SPA forces equal the plain per-window
+    runs bit for bit (same tiling) and the oracle to 1e-5; nonbond forces
+    match the oracle per window to 1e-5."""
+    wins = _mixed_box_windows()
+    m = batch.merge_windows(wins)
+    sl = batch.window_slices(m)
+    g = _gpu(m, bonded=False, us=False)
+    g.set_tiling(2, 3)
+    comp = {k: v.cpu().numpy() for k, v in g.isspa_components(step=4).items()}
+    nb = g.nonbond_components()
+    nbt = {k: v.cpu().numpy() for k, v in nb.items()}
+    g.close()
+    for w, (s, (a, b)) in enumerate(zip(wins, sl)):
+        print(f"window {w}: N={s.n_atoms} box {s.lbox}", flush=True)
+        p = _gpu(s, nonbond=False, bonded=False, us=False, system_id=w)
+        p.set_tiling(2, 3)
+        pc = p.isspa_components(step=4)
+        for k in ("lj", "pol", "cdd"):
+            assert np.array_equal(comp[k][a:b], pc[k].cpu().numpy()), (w, k)
+        p.close()
+        ref = po.isspa_total(s, step=4, sys_id=w)
+        for k in ("lj", "pol", "cdd"):
+            if np.abs(ref[k]).max() == 0.0:
+                assert np.abs(comp[k][a:b]).max() == 0.0
+                continue
+            err = max_abs_rel_rms(comp[k][a:b], ref[k])
+            print(f"  isspa {k}: rel RMS {err:.2e}", flush=True)
+            assert err <= 1e-5, (w, k, err)
+        if s.n_atoms > 1:
+            nref = po.nonbond(s)
+            for k in ("lj", "coul", "pair"):
+                if np.abs(nref[k]).max() == 0.0:
+                    continue
+                err = max_abs_rel_rms(nbt[k][a:b], nref[k])
+                print(f"  nonbond {k}: rel RMS {err:.2e}", flush=True)
+                assert err <= 1e-5, (w, k, err)
