@@ -58,6 +58,14 @@ int isspa_stream_sync(void *stream);
 /* milliseconds between two points of a stream: begin() records, end() syncs */
 int isspa_timer_begin(void **timer, void *stream);
 int isspa_timer_end(void *timer, void *stream, float *ms);
+/* a reusable pair of events: mark the begin and end of some work on a stream
+ * (no waiting), read the elapsed milliseconds later (waits for the end mark
+ * only), mark again for the next piece of work */
+int isspa_events_create(void **pair);
+int isspa_events_mark_begin(void *pair, void *stream);
+int isspa_events_mark_end(void *pair, void *stream);
+int isspa_events_elapsed(void *pair, float *ms);
+int isspa_events_destroy(void *pair);
 
 /* ------------------------------------------------------------------------- */
 /* Batched windows.  W independent systems (e.g. umbrella-sampling windows,  */
@@ -123,6 +131,12 @@ int isspa_model_destroy(isspa_model *m);
  * replaced by isspa_model_force_components. */
 int isspa_model_force(isspa_model *m, const float *xyzq4_d, float *f4_d, uint32_t step,
                       const uint32_t *step_d, void *stream);
+/* the same, and the IS-SPA force alone also stored (overwritten, w = 0) in
+ * f_isspa4_d by the same reduction kernel. Fills the isspaf_d argument of
+ * src/isspa_force_cuda.cu:331-360, which the reference zeroes
+ * (src/isspa_force_cuda.cu:343) and never accumulates. */
+int isspa_model_force_keep(isspa_model *m, const float *xyzq4_d, float *f4_d, float *f_isspa4_d,
+                           uint32_t step, const uint32_t *step_d, void *stream);
 /* Per-component forces in FP64 (double[N*3] each, overwritten): LJ-table term,
  * polarisation (dipole + quadrupole + octupole), constant-density dielectric.
  * These are the components of the reference's statistical fixtures. */

@@ -361,11 +361,14 @@ static int isspa_field_pass(isspa_model *m, const float *xyzq4_d, uint32_t step,
 }
 
 static int isspa_force_pass(isspa_model *m, const float *xyzq4_d, bool split, float4 *f,
-                            double *flj, double *fpol, double *fcdd, hipStream_t s)
+                            double *flj, double *fpol, double *fcdd, hipStream_t s,
+                            float4 *f_alone = nullptr)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
     HIPCHK(launch_force(force_args(m, x), m->npblk, split, s));
-    HIPCHK(launch_force_reduce(reduce_args(m, x, f, flj, fpol, fcdd), split, s));
+    ReduceArgs ra = reduce_args(m, x, f, flj, fpol, fcdd);
+    ra.f_alone = f_alone;
+    HIPCHK(launch_force_reduce(ra, split, s));
     return ISSPA_OK;
 }
 
@@ -376,6 +379,17 @@ extern "C" int isspa_model_force(isspa_model *m, const float *xyzq4_d, float *f4
     CHK(isspa_field_pass(m, xyzq4_d, step, step_d, S(stream)));
     return isspa_force_pass(m, xyzq4_d, false, reinterpret_cast<float4 *>(f4_d), nullptr, nullptr,
                             nullptr, S(stream));
+}
+
+extern "C" int isspa_model_force_keep(isspa_model *m, const float *xyzq4_d, float *f4_d,
+                                      float *f_isspa4_d, uint32_t step, const uint32_t *step_d,
+                                      void *stream)
+{
+    if (!m || !xyzq4_d || !f4_d || !f_isspa4_d)
+        return fail(ISSPA_ERR_ARG, "isspa_model_force_keep: null argument");
+    CHK(isspa_field_pass(m, xyzq4_d, step, step_d, S(stream)));
+    return isspa_force_pass(m, xyzq4_d, false, reinterpret_cast<float4 *>(f4_d), nullptr, nullptr,
+                            nullptr, S(stream), reinterpret_cast<float4 *>(f_isspa4_d));
 }
 
 extern "C" int isspa_model_force_components(isspa_model *m, const float *xyzq4_d,

@@ -110,6 +110,50 @@ extern "C" int isspa_timer_end(void *timer, void *stream, float *ms)
     return ISSPA_OK;
 }
 
+extern "C" int isspa_events_create(void **pair)
+{
+    if (!pair) return isspa_fail(ISSPA_ERR_ARG, "isspa_events_create: null argument");
+    timer_pair *t = new (std::nothrow) timer_pair;
+    if (!t) return isspa_fail(ISSPA_ERR_ALLOC, "event pair allocation");
+    hipError_t e = hipEventCreate(&t->a);
+    if (e == hipSuccess) e = hipEventCreate(&t->b);
+    if (e != hipSuccess) {
+        delete t;
+        return isspa_fail(ISSPA_ERR_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    *pair = t;
+    return ISSPA_OK;
+}
+extern "C" int isspa_events_mark_begin(void *pair, void *stream)
+{
+    if (!pair) return isspa_fail(ISSPA_ERR_ARG, "null event pair");
+    HIPCHK(hipEventRecord(((timer_pair *)pair)->a, S(stream)));
+    return ISSPA_OK;
+}
+extern "C" int isspa_events_mark_end(void *pair, void *stream)
+{
+    if (!pair) return isspa_fail(ISSPA_ERR_ARG, "null event pair");
+    HIPCHK(hipEventRecord(((timer_pair *)pair)->b, S(stream)));
+    return ISSPA_OK;
+}
+extern "C" int isspa_events_elapsed(void *pair, float *ms)
+{
+    if (!pair || !ms) return isspa_fail(ISSPA_ERR_ARG, "isspa_events_elapsed: null argument");
+    timer_pair *t = (timer_pair *)pair;
+    HIPCHK(hipEventSynchronize(t->b));
+    HIPCHK(hipEventElapsedTime(ms, t->a, t->b));
+    return ISSPA_OK;
+}
+extern "C" int isspa_events_destroy(void *pair)
+{
+    if (!pair) return ISSPA_OK;
+    timer_pair *t = (timer_pair *)pair;
+    (void)hipEventDestroy(t->a);
+    (void)hipEventDestroy(t->b);
+    delete t;
+    return ISSPA_OK;
+}
+
 extern "C" int isspa_default_constants(float *e0, float *p0, float *q0, float *o0, float *ep)
 {
     // src/constants.h:6-13, evaluated in FP32 (volatile keeps the compiler from

@@ -576,11 +576,15 @@ __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
     const double vt = (double)a.vtot[a.itype[i]];
     const double qv = -(double)x.w * vt;
     if (!SPLIT) {
+        const float fx = (float)(qv * s[0] - vt * s[3]);
+        const float fy = (float)(qv * s[1] - vt * s[4]);
+        const float fz = (float)(qv * s[2] - vt * s[5]);
         float4 f = a.f[i];
-        f.x += (float)(qv * s[0] - vt * s[3]);
-        f.y += (float)(qv * s[1] - vt * s[4]);
-        f.z += (float)(qv * s[2] - vt * s[5]);
+        f.x += fx;
+        f.y += fy;
+        f.z += fz;
         a.f[i] = f;
+        if (a.f_alone) a.f_alone[i] = make_float4(fx, fy, fz, 0.0f);
     } else {
         for (int c = 0; c < 3; ++c) {
             a.f_pol[3 * i + c] = qv * s[c];

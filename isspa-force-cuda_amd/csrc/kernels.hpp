@@ -119,6 +119,7 @@ struct ReduceArgs {
     const int *itype;
     const float *vtot;    // per type
     float4 *f;            // += total (may be null)
+    float4 *f_alone;      // = total, the IS-SPA force kept apart (may be null)
     double *f_lj, *f_pol, *f_cdd;  // component outputs (split mode)
 };
 

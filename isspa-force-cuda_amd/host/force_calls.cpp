@@ -1,6 +1,6 @@
 // The reference's wrapper functions over the C ABI (see force_calls.h).
-// Device models are built on first use from the host classes' arrays, each
-// call is timed with a pair of events on the null stream and returns ms.
+// Device models are built on first use from the host classes' arrays; each
+// call is timed with events on the null stream (LagTimer below).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -18,14 +18,33 @@ void isspa_check(int status, const char *what)
 
 namespace {
 
-struct Timer {
-    void *t = nullptr;
-    Timer() { isspa_check(isspa_timer_begin(&t, nullptr), "isspa_timer_begin"); }
-    float stop()
+// Per-wrapper timing without a synchronisation per call: two reusable event
+// pairs alternate, and each call returns the duration of the wrapper's
+// previous call (waiting only for that call's end mark, which the device has
+// normally passed by then).  Summed over a run this is the reference's
+// per-call timing less the last call of each wrapper.
+struct LagTimer {
+    void *pair[2] = {nullptr, nullptr};
+    bool pending[2] = {false, false};
+    int k = 0;
+    void begin()
     {
+        if (!pair[0]) {
+            isspa_check(isspa_events_create(&pair[0]), "isspa_events_create");
+            isspa_check(isspa_events_create(&pair[1]), "isspa_events_create");
+        }
+        isspa_check(isspa_events_mark_begin(pair[k], nullptr), "isspa_events_mark_begin");
+    }
+    float end()
+    {
+        isspa_check(isspa_events_mark_end(pair[k], nullptr), "isspa_events_mark_end");
+        pending[k] = true;
+        k ^= 1;
         float ms = 0.0f;
-        isspa_check(isspa_timer_end(t, nullptr, &ms), "isspa_timer_end");
-        t = nullptr;
+        if (pending[k]) {
+            isspa_check(isspa_events_elapsed(pair[k], &ms), "isspa_events_elapsed");
+            pending[k] = false;
+        }
         return ms;
     }
 };
@@ -74,20 +93,20 @@ float isspa_force_cuda(float4 *xyz_d, float4 *f_d, float4 *isspaf_d, isspa &is, 
         exit(3);
     }
     (void)nAtoms;
-    Timer t;
+    static LagTimer t;
+    t.begin();
     if (isspaf_d) {
-        // the IS-SPA force alone is kept in isspaf (the reference zeroes it and
-        // never fills it, quirk Q17), then folded into f
-        isspa_check(isspa_zero(isspaf_d, (size_t)is.nAtoms * sizeof(float4), nullptr), "isspa_zero");
-        isspa_check(isspa_model_force(is.model, (const float *)xyz_d, (float *)isspaf_d, is.step,
-                                      nullptr, nullptr), "isspa_model_force");
-        isspa_check(isspa_add4((float *)f_d, (const float *)isspaf_d, is.nAtoms, nullptr), "isspa_add4");
+        // the IS-SPA force alone is also kept in isspaf (the reference zeroes
+        // it and never fills it, quirk Q17); one reduction writes both
+        isspa_check(isspa_model_force_keep(is.model, (const float *)xyz_d, (float *)f_d,
+                                           (float *)isspaf_d, is.step, nullptr, nullptr),
+                    "isspa_model_force_keep");
     } else {
         isspa_check(isspa_model_force(is.model, (const float *)xyz_d, (float *)f_d, is.step,
                                       nullptr, nullptr), "isspa_model_force");
     }
     ++is.step;
-    return t.stop();
+    return t.end();
 }
 
 // ------------------------------------------------------------ nonbond -----
@@ -113,10 +132,11 @@ float nonbond_force_cuda(atom &atoms, isspa &is, int nAtoms)
                                          is.nTypes, is.rmax_h, ep, atoms.lbox, &atoms.nonbond),
                     "nonbond_model_create");
     }
-    Timer t;
+    static LagTimer t;
+    t.begin();
     isspa_check(nonbond_force(atoms.nonbond, (const float *)atoms.pos_d, (float *)atoms.for_d,
                               nullptr), "nonbond_force");
-    return t.stop();
+    return t.end();
 }
 
 // ------------------------------------------------------------ bonded ------
@@ -139,10 +159,11 @@ float bond_force_cuda(float4 *xyz_d, float4 *f_d, int nAtoms, float lbox, bond &
                                         0, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, 0,
                                         nullptr, nullptr, nullptr, nullptr, lbox, &bonds.model),
                     "bonded_model_create(bonds)");
-    Timer t;
+    static LagTimer t;
+    t.begin();
     isspa_check(bonded_force(bonds.model, ISSPA_TERM_BOND, (const float *)xyz_d, (float *)f_d,
                              nullptr, nullptr), "bonded_force(bonds)");
-    return t.stop();
+    return t.end();
 }
 
 float angle_force_cuda(float4 *xyz_d, float4 *f_d, int nAtoms, float lbox, angle &angles)
@@ -154,10 +175,11 @@ float angle_force_cuda(float4 *xyz_d, float4 *f_d, int nAtoms, float lbox, angle
                                         0, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr,
                                         lbox, &angles.model),
                     "bonded_model_create(angles)");
-    Timer t;
+    static LagTimer t;
+    t.begin();
     isspa_check(bonded_force(angles.model, ISSPA_TERM_ANGLE, (const float *)xyz_d, (float *)f_d,
                              nullptr, nullptr), "bonded_force(angles)");
-    return t.stop();
+    return t.end();
 }
 
 float dih_force_cuda(atom &atoms, dih &dihs, float lbox)
@@ -174,22 +196,24 @@ float dih_force_cuda(atom &atoms, dih &dihs, float lbox)
                                         B.data(), lbox, &dihs.model),
                     "bonded_model_create(dihedrals)");
     }
-    Timer t;
+    static LagTimer t;
+    t.begin();
     isspa_check(bonded_force(dihs.model, ISSPA_TERM_DIH, (const float *)atoms.pos_d,
                              (float *)atoms.for_d, nullptr, nullptr), "bonded_force(dihedrals)");
-    return t.stop();
+    return t.end();
 }
 
 // ------------------------------------------------------------ integrate ---
 float leapfrog_cuda(atom &atoms, config &c)
 {
-    Timer t;
+    static LagTimer t;
+    t.begin();
     isspa_check(leapfrog_andersen(atoms.nAtoms, (float *)atoms.pos_d, (float *)atoms.vel_d,
                                   (const float *)atoms.for_d, c.T, c.dt, c.pnu, c.lbox,
                                   (uint64_t)(uint32_t)atoms.seed, 0u, atoms.step, nullptr, nullptr),
                 "leapfrog_andersen");
     ++atoms.step;
-    return t.stop();
+    return t.end();
 }
 
 // ------------------------------------------------------------ umbrella ----
@@ -205,10 +229,11 @@ float us_force_cuda(float4 *xyz_d, float4 *f_d, us &bias, float lbox, int nAtoms
         isspa_check(us_model_create(nAtoms, bias.totalBiasAtoms, (const int *)bias.atomList_h,
                                     bias.mass_h, bias.kumb_h, bias.x0, lbox, &bias.model),
                     "us_model_create");
-    Timer t;
+    static LagTimer t;
+    t.begin();
     isspa_check(us_force(bias.model, (const float *)xyz_d, (float *)f_d, bias.groupComPos_d,
                          nullptr), "us_force");
-    return t.stop();
+    return t.end();
 }
 
 extern "C" void init_rand_states(void *, int, int)

@@ -68,12 +68,18 @@ _SIGS = {
     "isspa_stream_sync": ([vp], i32),
     "isspa_timer_begin": ([C.POINTER(vp), vp], i32),
     "isspa_timer_end": ([vp, vp, fp], i32),
+    "isspa_events_create": ([C.POINTER(vp)], i32),
+    "isspa_events_mark_begin": ([vp, vp], i32),
+    "isspa_events_mark_end": ([vp, vp], i32),
+    "isspa_events_elapsed": ([vp, fp], i32),
+    "isspa_events_destroy": ([vp], i32),
     "isspa_default_constants": ([fp, fp, fp, fp, fp], i32),
     "isspa_model_create": ([C.POINTER(ModelDesc), vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
     "isspa_model_create_batch": ([C.POINTER(ModelDesc), C.POINTER(Windows), vp, vp, vp, vp, vp,
                                   C.POINTER(vp)], i32),
     "isspa_model_destroy": ([vp], i32),
     "isspa_model_force": ([vp, vp, vp, u32, vp, vp], i32),
+    "isspa_model_force_keep": ([vp, vp, vp, vp, u32, vp, vp], i32),
     "isspa_model_force_components": ([vp, vp, vp, vp, vp, u32, vp], i32),
     "isspa_model_get_field": ([vp, vp, vp, vp, vp, vp, vp], i32),
     "isspa_model_profile": ([vp, vp, i32, fp, vp], i32),

@@ -159,6 +159,13 @@ class GpuSystem:
                                          self.stream()), "isspa_model_force")
         return f
 
+    def isspa_force_keep(self, f: torch.Tensor, f_isspa: torch.Tensor, step: int = 0):
+        """Adds the IS-SPA force into f and stores it alone in f_isspa, in one
+        reduction (the reference wrapper's isspaf_d argument)."""
+        check(capi.lib.isspa_model_force_keep(self.isspa, ptr(self.x), ptr(f), ptr(f_isspa), step,
+                                              None, self.stream()), "isspa_model_force_keep")
+        return f, f_isspa
+
     def isspa_components(self, step: int = 0):
         n = self.system.n_atoms
         out = [torch.zeros((n, 3), dtype=torch.float64, device=self.dev) for _ in range(3)]

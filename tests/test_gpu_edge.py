@@ -103,6 +103,28 @@ def test_isspa_oversized_tilings_are_invariant(case):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["one_atom_nmc7", "264_atoms_nmc1"])
+def test_force_keep_matches_force_and_keeps_the_isspa_part(case):
+    """isspa_model_force_keep adds into f exactly what isspa_model_force adds
+    and stores that same FP32 force alone (w = 0) in the second array."""
+    import torch
+    s = CASES[case]()
+    g = _gpu(s, nonbond=False, bonded=False)
+    gen = torch.Generator().manual_seed(5)
+    f0 = torch.randn((s.n_atoms, 4), generator=gen).to("cuda:0")
+    ref = g.isspa_force(step=2, f=f0.clone())
+    alone = torch.zeros_like(f0)
+    g.isspa_force(step=2, f=alone)
+    f = f0.clone()
+    kept = torch.full_like(f0, 7.0)
+    g.isspa_force_keep(f, kept, step=2)
+    torch.cuda.synchronize()
+    assert torch.equal(f, ref)
+    assert torch.equal(kept[:, :3], alone[:, :3])
+    assert torch.all(kept[:, 3] == 0.0)
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["two_atoms_nmc3", "66_atoms_nmc2", "264_atoms_nmc1"])
 def test_nonbond_and_bonded_edge_sizes(case):
     s = CASES[case]()
