@@ -180,9 +180,9 @@ static int diff_prepare(isspa_diff *m, const float *xyz4_d, const double *q_d,
 static int diff_isspa_forward(isspa_diff *m, const double *q_d, double *f_d, hipStream_t s)
 {
     isspa_model *im = m->im;
-    HIPCHK(launch_pbc_flag(im->wins, im->lay.W, m->xq, im->rmax_all, im->pbc_flag, s));
     HIPCHK(launch_mc_points(im->wins, im->atom_win, m->xq, im->itype, im->rmax, im->rmax2,
-                            im->d.n_atoms, im->d.n_mc, im->d.seed, m->step, nullptr, im->pts, s));
+                            im->d.n_atoms, im->d.n_mc, im->d.seed, m->step, nullptr, im->pts, s,
+                            im->lay.W, im->rmax_all, im->pbc_flag));
     FieldArgs fa = field_args(im, m->xq);
     fa.dtypes = m->dtypes;
     HIPCHK(launch_field(fa, im->nfblk, s));

@@ -352,9 +352,9 @@ static int isspa_field_pass(isspa_model *m, const float *xyzq4_d, uint32_t step,
                             const uint32_t *step_d, hipStream_t s)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
-    HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, m->rmax_all, m->pbc_flag, s));
     HIPCHK(launch_mc_points(m->wins, m->atom_win, x, m->itype, m->rmax, m->rmax2, m->d.n_atoms,
-                            m->d.n_mc, m->d.seed, step, step_d, m->pts, s));
+                            m->d.n_mc, m->d.seed, step, step_d, m->pts, s, m->lay.W, m->rmax_all,
+                            m->pbc_flag));
     HIPCHK(launch_field(field_args(m, x), m->nfblk, s));
     HIPCHK(launch_epilogue(epilogue_args(m), s));
     return ISSPA_OK;

@@ -25,6 +25,7 @@
 #include "device_common.hpp"
 #include "diff_device.hpp"
 #include "kernels.hpp"
+#include "pbc_device.hpp"
 
 namespace isspa {
 
@@ -66,14 +67,23 @@ __device__ __forceinline__ float point_coord(float x, float R, int v)
 // A.1 MC points (src/isspa_force_cuda.cu:73-107), Philox instead of curand.
 // Point m of window w (owner m / nMC) is drawn with counter
 // {m, draw, step, sys_w}.  .w of each point carries the owner's rmax^2.
+// The first nflag blocks (one per window, none when nflag = 0) compute
+// instead the per-window minimum-image flags that the field and force passes
+// read, which saves a launch.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_mc_points(
     const WinDesc *__restrict__ wins, const int *__restrict__ atom_win,
     const float4 *__restrict__ xyzq, const int *__restrict__ itype, const float *__restrict__ rmax,
     const float *__restrict__ rmax2, int ntot, int nmc, uint64_t seed, uint32_t step_arg,
-    const uint32_t *__restrict__ step_d, float4 *__restrict__ pts)
+    const uint32_t *__restrict__ step_d, float4 *__restrict__ pts, int nflag, float rmax_all,
+    int *flag)
 {
-    const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if ((int)blockIdx.x < nflag) {
+        const WinDesc W = wins[blockIdx.x];
+        pbc_flag_block(W, xyzq, rmax_all, flag + blockIdx.x);
+        return;
+    }
+    const long long gp = (long long)(blockIdx.x - nflag) * kBlock + threadIdx.x;
     if (gp >= (long long)ntot * nmc) return;
     const uint32_t step = step_d ? *step_d : step_arg;
     const int k = (int)(gp / nmc);
@@ -387,13 +397,21 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
                                            const DiffFParams &fp, TileSums &ts, ForceAcc &acc)
 {
     float &sx = ts.sx, &sy = ts.sy, &sz = ts.sz;
-    // 32-bit point indices within the window keep the loop test on the SALU
+    // the next point's record (wave-uniform) is fetched while this one is
+    // evaluated, which hides the scalar load latency when few waves share a
+    // SIMD; 32-bit point indices within the window keep the loop test on the
+    // SALU
+    float4 n0 = rec[4 * m0 + 0], n1 = rec[4 * m0 + 1], n2 = rec[4 * m0 + 2],
+           n3 = rec[4 * m0 + 3], na = aux[m0];
 #pragma unroll 2
     for (int m = m0; m < m1; ++m) {
-        const float4 q0 = rec[4 * m + 0];  // wave-uniform
-        const float4 q1 = rec[4 * m + 1];
-        const float4 q2 = rec[4 * m + 2];
-        const float4 q3 = rec[4 * m + 3];
+        const float4 q0 = n0, q1 = n1, q2 = n2, q3 = n3, c4 = na;
+        const int mn = min(m + 1, m1 - 1);
+        n0 = rec[4 * mn + 0];
+        n1 = rec[4 * mn + 1];
+        n2 = rec[4 * mn + 2];
+        n3 = rec[4 * mn + 3];
+        na = aux[mn];
         const float Ex = q0.w, Ey = q1.x, Ez = q1.y;
         const float Vx = q1.z, Vy = q1.w, Vz = q2.x;
         const float Dx = q2.y, Dy = q2.z, Dz = q2.w;
@@ -431,7 +449,6 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
         } else {
             // polarisation alone: alpha_p r - (d^-3 A1 + k) Ehat, alpha_p
             // being alpha without the E0c.r part
-            const float4 c4 = aux[m];
             const float A1 = c4.w;
             const float ap =
                 __builtin_fmaf(A2, P2, __builtin_fmaf(A3, P3p * t, (3.0f * A1) * e)) * rinv5;
@@ -464,7 +481,7 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
             ts.lz = __builtin_fmaf(fl, rz, ts.lz);
             if (!SPLIT) {
                 // remove the CDD part taken in the main path
-                const float3 c = cdd_term(aux[m], rx, ry, rz, rinv3, rinv5);
+                const float3 c = cdd_term(c4, rx, ry, rz, rinv3, rinv5);
                 sx -= c.x;
                 sy -= c.y;
                 sz -= c.z;
@@ -561,7 +578,22 @@ __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
     if (valid) {
         const WinDesc W = a.wins[a.atom_win[i]];
         const long long base = W.ppo + (i - W.a0);
-        for (int c = sl; c < W.chunks; c += kRedSlices) {
+        // four chunks' loads in flight at a time, added in chunk order
+        int c = sl;
+        for (; c + 3 * kRedSlices < W.chunks; c += 4 * kRedSlices) {
+            double v[4][comps];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long long o = base + (long long)(c + u * kRedSlices) * W.n;
+#pragma unroll
+                for (int k = 0; k < comps; ++k) v[u][k] = a.part[k * a.C + o];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int k = 0; k < comps; ++k) s[k] += v[u][k];
+        }
+        for (; c < W.chunks; c += kRedSlices) {
             const long long o = base + (long long)c * W.n;
             for (int k = 0; k < comps; ++k) s[k] += a.part[k * a.C + o];
         }
@@ -602,12 +634,14 @@ static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlo
 hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const float4 *xyzq,
                             const int *itype, const float *rmax, const float *rmax2, int ntot,
                             int nmc, uint64_t seed, uint32_t step, const uint32_t *step_d,
-                            float4 *pts, hipStream_t s)
+                            float4 *pts, hipStream_t s, int nwin, float rmax_all, int *flag)
 {
     const long long P = (long long)ntot * nmc;
-    if (P == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P)), dim3(kBlock), 0, s, wins, atom_win, xyzq,
-                       itype, rmax, rmax2, ntot, nmc, seed, step, step_d, pts);
+    const int nflag = flag ? nwin : 0;
+    if (P == 0 && nflag == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P) + nflag), dim3(kBlock), 0, s, wins,
+                       atom_win, xyzq, itype, rmax, rmax2, ntot, nmc, seed, step, step_d, pts,
+                       nflag, rmax_all, flag);
     return hipGetLastError();
 }
 

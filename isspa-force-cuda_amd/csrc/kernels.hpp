@@ -126,7 +126,8 @@ struct ReduceArgs {
 hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const float4 *xyzq,
                             const int *itype, const float *rmax, const float *rmax2, int ntot,
                             int nmc, uint64_t seed, uint32_t step, const uint32_t *step_d,
-                            float4 *pts, hipStream_t s);
+                            float4 *pts, hipStream_t s, int nwin = 0, float rmax_all = 0.0f,
+                            int *flag = nullptr);
 hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s);
 hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s);

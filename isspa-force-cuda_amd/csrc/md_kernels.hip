@@ -8,6 +8,7 @@
 // The all-pairs, integrator and restraint kernels are window-aware (kernels.hpp).
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "pbc_device.hpp"
 
 namespace isspa {
 
@@ -565,44 +566,13 @@ hipError_t launch_us(int nres, const int *boff, const int2 *atoms, const float *
 // ---------------------------------------------------------------------------
 // utilities
 // ---------------------------------------------------------------------------
-// flag[w] = 1 when some pair separation of window w may exceed half a box
-// (minimum image needed); 0 when (extent + rmax + margin) < L/2 on every axis,
-// in which case the single-image minimum image is the identity for every
-// atom-atom and point-atom pair and the kernels take their shift-free loops.
+// flag[w] for every window w: one block per window
 __global__ __launch_bounds__(256) void k_pbc_flag(const WinDesc *__restrict__ wins,
                                                   const float4 *__restrict__ xyzq,
                                                   float rmax_all, int *flag)
 {
-    __shared__ float lo[3][256], hi[3][256];
     const WinDesc W = wins[blockIdx.x];
-    float l[3] = {3.0e38f, 3.0e38f, 3.0e38f}, u[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
-    for (int i = W.a0 + threadIdx.x; i < W.a0 + W.n; i += 256) {
-        const float4 x = xyzq[i];
-        l[0] = fminf(l[0], x.x); u[0] = fmaxf(u[0], x.x);
-        l[1] = fminf(l[1], x.y); u[1] = fmaxf(u[1], x.y);
-        l[2] = fminf(l[2], x.z); u[2] = fmaxf(u[2], x.z);
-    }
-    for (int k = 0; k < 3; ++k) {
-        lo[k][threadIdx.x] = l[k];
-        hi[k][threadIdx.x] = u[k];
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int k = 0; k < 3; ++k) {
-                lo[k][threadIdx.x] = fminf(lo[k][threadIdx.x], lo[k][threadIdx.x + w]);
-                hi[k][threadIdx.x] = fmaxf(hi[k][threadIdx.x], hi[k][threadIdx.x + w]);
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        int need = 0;
-        for (int k = 0; k < 3; ++k) {
-            const float ext = hi[k][0] - lo[k][0];
-            if (!(ext + rmax_all + 1.0e-2f < W.h)) need = 1;  // NaN -> need
-        }
-        flag[blockIdx.x] = need;
-    }
+    pbc_flag_block(W, xyzq, rmax_all, flag + blockIdx.x);
 }
 
 hipError_t launch_pbc_flag(const WinDesc *wins, int W, const float4 *xyzq, float rmax_all,

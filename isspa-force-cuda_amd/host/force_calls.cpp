@@ -1,6 +1,6 @@
 // The reference's wrapper functions over the C ABI (see force_calls.h).
-// Device models are built on first use from the host classes' arrays; each
-// call is timed with events on the null stream (LagTimer below).
+// Device models are built on first use from the host classes' arrays; one
+// call in 16 is timed with events on the null stream (LagTimer below).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -18,34 +18,39 @@ void isspa_check(int status, const char *what)
 
 namespace {
 
-// Per-wrapper timing without a synchronisation per call: two reusable event
-// pairs alternate, and each call returns the duration of the wrapper's
-// previous call (waiting only for that call's end mark, which the device has
-// normally passed by then).  Summed over a run this is the reference's
-// per-call timing less the last call of each wrapper.
+// Per-wrapper timing without a synchronisation per call.  A timed event pair
+// around a launch makes the device drain before and after it (about 4 us each
+// on MI355X, more than a small system's kernels take), so only one call in
+// kPeriod is bracketed by events; every call returns the duration of the
+// latest bracketed call that has been read back.  That one is read at the next
+// bracketed call, kPeriod calls later, when the device has long passed its
+// end mark.  Summed over a run this estimates the reference's per-call timing
+// (src/total_force_cuda.cpp:109-153 adds each call's milliseconds) from one
+// call in kPeriod, lagged by kPeriod calls.
 struct LagTimer {
-    void *pair[2] = {nullptr, nullptr};
-    bool pending[2] = {false, false};
-    int k = 0;
+    static constexpr unsigned kPeriod = 16;
+    void *pair = nullptr;
+    bool pending = false, marked = false;
+    unsigned n = 0;
+    float last = 0.0f;
     void begin()
     {
-        if (!pair[0]) {
-            isspa_check(isspa_events_create(&pair[0]), "isspa_events_create");
-            isspa_check(isspa_events_create(&pair[1]), "isspa_events_create");
+        marked = n++ % kPeriod == 0;
+        if (!marked) return;
+        if (!pair) isspa_check(isspa_events_create(&pair), "isspa_events_create");
+        if (pending) {
+            isspa_check(isspa_events_elapsed(pair, &last), "isspa_events_elapsed");
+            pending = false;
         }
-        isspa_check(isspa_events_mark_begin(pair[k], nullptr), "isspa_events_mark_begin");
+        isspa_check(isspa_events_mark_begin(pair, nullptr), "isspa_events_mark_begin");
     }
     float end()
     {
-        isspa_check(isspa_events_mark_end(pair[k], nullptr), "isspa_events_mark_end");
-        pending[k] = true;
-        k ^= 1;
-        float ms = 0.0f;
-        if (pending[k]) {
-            isspa_check(isspa_events_elapsed(pair[k], &ms), "isspa_events_elapsed");
-            pending[k] = false;
+        if (marked) {
+            isspa_check(isspa_events_mark_end(pair, nullptr), "isspa_events_mark_end");
+            pending = true;
         }
-        return ms;
+        return last;
     }
 };
 
