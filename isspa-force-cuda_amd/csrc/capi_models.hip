@@ -803,11 +803,9 @@ extern "C" int bonded_model_destroy(bonded_model *m)
     return ISSPA_OK;
 }
 
-extern "C" int bonded_force(bonded_model *m, int terms, const float *xyzq4_d, float *f4_d,
-                            double *energies_d, void *stream)
+// the per-term kernel: every term's forces into its contribution slots
+static int bonded_terms(bonded_model *m, int terms, const float *xyzq4_d, hipStream_t s)
 {
-    if (!m || !xyzq4_d || !f4_d) return fail(ISSPA_ERR_ARG, "bonded_force: null argument");
-    hipStream_t s = S(stream);
     BondedArgs a{};
     a.xyzq = reinterpret_cast<const float4 *>(xyzq4_d);
     a.L = m->lbox; a.h = 0.5f * m->lbox;
@@ -818,6 +816,15 @@ extern "C" int bonded_force(bonded_model *m, int terms, const float *xyzq4_d, fl
     a.sc14 = m->sc14; a.ityp = m->ityp; a.lj = m->lj; a.T = m->T;
     a.contrib = m->contrib; a.energy = m->energy; a.terms = terms;
     HIPCHK(launch_bonded_terms(a, s));
+    return ISSPA_OK;
+}
+
+extern "C" int bonded_force(bonded_model *m, int terms, const float *xyzq4_d, float *f4_d,
+                            double *energies_d, void *stream)
+{
+    if (!m || !xyzq4_d || !f4_d) return fail(ISSPA_ERR_ARG, "bonded_force: null argument");
+    hipStream_t s = S(stream);
+    CHK(bonded_terms(m, terms, xyzq4_d, s));
     HIPCHK(launch_bonded_gather(m->contrib, m->gptr, m->gidx, m->n,
                                 reinterpret_cast<float4 *>(f4_d), s));
     if (energies_d) HIPCHK(launch_bonded_energy(m->energy, m->nb, m->na, m->nd, energies_d, s));
@@ -1053,10 +1060,13 @@ extern "C" int md_engine_set_step(md_engine *e, uint32_t step, void *stream)
     return ISSPA_OK;
 }
 
+// Every term writes its own partials or contribution slots; one assembly
+// launch then forms f = bonded + IS-SPA + all-pairs (no zeroing launch), bit
+// for bit what zero + bonded_force + isspa_model_force + nonbond_force give,
+// and the restraint adds its force last.
 static int engine_forces(md_engine *e, hipStream_t s)
 {
-    float4 *f4 = reinterpret_cast<float4 *>(e->f);
-    HIPCHK(launch_zero(e->f, (size_t)e->n * sizeof(float4), s));
+    const float4 *x4 = reinterpret_cast<const float4 *>(e->x);
     const bool fork = e->side != nullptr;
     if (fork) {
         // all-pairs partials beside the IS-SPA passes: they read x and write
@@ -1066,17 +1076,31 @@ static int engine_forces(md_engine *e, hipStream_t s)
         CHK(nonbond_partials(e->nb, e->x, false, e->side));
         HIPCHK(hipEventRecord(e->ev_join, e->side));
     }
-    if (e->bonded) CHK(bonded_force(e->bonded, ISSPA_TERM_ALL, e->x, e->f, nullptr, s));
+    AssembleArgs a{};
+    a.n = e->n;
+    a.f = reinterpret_cast<float4 *>(e->f);
+    if (e->bonded) {
+        CHK(bonded_terms(e->bonded, ISSPA_TERM_ALL, e->x, s));
+        a.contrib = e->bonded->contrib;
+        a.gptr = e->bonded->gptr;
+        a.gidx = e->bonded->gidx;
+    }
     if (e->isspa) {
         CHK(isspa_field_pass(e->isspa, e->x, 0u, e->step_d, s));
-        CHK(isspa_force_pass(e->isspa, e->x, false, f4, nullptr, nullptr, nullptr, s));
+        HIPCHK(launch_force(force_args(e->isspa, x4), e->isspa->npblk, false, s));
+        a.isspa = reduce_args(e->isspa, x4, nullptr, nullptr, nullptr, nullptr);
     }
-    if (fork) {
+    if (fork)
         HIPCHK(hipStreamWaitEvent(s, e->ev_join, 0));
-        CHK(nonbond_reduce(e->nb, false, f4, nullptr, nullptr, nullptr, s));
-    } else if (e->nb) {
-        CHK(nonbond_force(e->nb, e->x, e->f, s));
+    else if (e->nb)
+        CHK(nonbond_partials(e->nb, e->x, false, s));
+    if (e->nb) {
+        a.nb_wins = e->nb->wins;
+        a.nb_atom_win = e->nb->atom_win;
+        a.nb_part = e->nb->part;
+        a.nb_B = e->nb->B;
     }
+    HIPCHK(launch_assemble_forces(a, s));
     if (e->us) CHK(us_force(e->us, e->x, e->f, e->com_d, s));
     return ISSPA_OK;
 }

@@ -26,6 +26,7 @@
 #include "diff_device.hpp"
 #include "kernels.hpp"
 #include "pbc_device.hpp"
+#include "reduce_device.hpp"
 
 namespace isspa {
 
@@ -556,74 +557,97 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     }
 }
 
-// F_i = -q_i vtot_i S_e - vtot_i S_lj over the chunk partials.  A block owns
-// kRedAtoms atoms and kRedSlices chunk slices: slice t sums chunks t, t + S,
-// t + 2S, ... in order, then slice 0 adds slices 1..S-1 in order, so the sum
-// is in a fixed order independent of scheduling.  The slices give the kernel
-// S times the waves of one lane per atom, which small systems need to hide
-// the load latency.
-constexpr int kRedAtoms = 32;
-constexpr int kRedSlices = kBlock / kRedAtoms;
-
+// F_i = -q_i vtot_i S_e - vtot_i S_lj over the chunk partials, summed in the
+// fixed slice order of isspa_slice_sum (32 atoms x 8 chunk slices a block).
 template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
 {
     constexpr int comps = SPLIT ? 9 : 6;
-    __shared__ double red[kRedSlices - 1][comps][kRedAtoms];
-    const int al = threadIdx.x % kRedAtoms, sl = threadIdx.x / kRedAtoms;
-    const int i = blockIdx.x * kRedAtoms + al;
+    __shared__ double red[kSumSlices - 1][comps][kSumAtoms];
+    const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
+    const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < a.ntot;
     double s[comps];
     for (int k = 0; k < comps; ++k) s[k] = 0.0;
-    if (valid) {
-        const WinDesc W = a.wins[a.atom_win[i]];
-        const long long base = W.ppo + (i - W.a0);
-        // four chunks' loads in flight at a time, added in chunk order
-        int c = sl;
-        for (; c + 3 * kRedSlices < W.chunks; c += 4 * kRedSlices) {
-            double v[4][comps];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const long long o = base + (long long)(c + u * kRedSlices) * W.n;
-#pragma unroll
-                for (int k = 0; k < comps; ++k) v[u][k] = a.part[k * a.C + o];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int k = 0; k < comps; ++k) s[k] += v[u][k];
-        }
-        for (; c < W.chunks; c += kRedSlices) {
-            const long long o = base + (long long)c * W.n;
-            for (int k = 0; k < comps; ++k) s[k] += a.part[k * a.C + o];
-        }
-    }
+    if (valid) isspa_slice_sum<comps>(a, i, sl, s);
     if (sl > 0)
         for (int k = 0; k < comps; ++k) red[sl - 1][k][al] = s[k];
     __syncthreads();
     if (sl != 0 || !valid) return;
-    for (int t = 0; t < kRedSlices - 1; ++t)
+    for (int t = 0; t < kSumSlices - 1; ++t)
         for (int k = 0; k < comps; ++k) s[k] += red[t][k][al];
-    const float4 x = a.xyzq[i];
-    const double vt = (double)a.vtot[a.itype[i]];
-    const double qv = -(double)x.w * vt;
     if (!SPLIT) {
-        const float fx = (float)(qv * s[0] - vt * s[3]);
-        const float fy = (float)(qv * s[1] - vt * s[4]);
-        const float fz = (float)(qv * s[2] - vt * s[5]);
+        const float3 fi = isspa_atom_force(a, i, s);
         float4 f = a.f[i];
-        f.x += fx;
-        f.y += fy;
-        f.z += fz;
+        f.x += fi.x;
+        f.y += fi.y;
+        f.z += fi.z;
         a.f[i] = f;
-        if (a.f_alone) a.f_alone[i] = make_float4(fx, fy, fz, 0.0f);
+        if (a.f_alone) a.f_alone[i] = make_float4(fi.x, fi.y, fi.z, 0.0f);
     } else {
+        const float4 x = a.xyzq[i];
+        const double vt = (double)a.vtot[a.itype[i]];
+        const double qv = -(double)x.w * vt;
         for (int c = 0; c < 3; ++c) {
             a.f_pol[3 * i + c] = qv * s[c];
             a.f_lj[3 * i + c] = -vt * s[3 + c];
             a.f_cdd[3 * i + c] = qv * s[6 + c];
         }
     }
+}
+
+// The MD engine's per-step force: f = ((bonded + IS-SPA) + all-pairs) per atom,
+// each term summed by the same helper and in the same order as its own
+// reduction kernel, and added in the order the separate calls add them into a
+// zeroed f, so the result equals zero + gather + reduce + chunk reduction bit
+// for bit in one launch.  The last slice also sums the all-pairs partials.
+__global__ __launch_bounds__(kBlock) void k_assemble_forces(AssembleArgs a)
+{
+    constexpr int nc = 9;  // bonded xyz, IS-SPA 6
+    __shared__ double red[kSumSlices - 1][nc][kSumAtoms];
+    __shared__ double nbs[3][kSumAtoms];
+    const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
+    const int i = blockIdx.x * kSumAtoms + al;
+    const bool valid = i < a.n;
+    double s[nc];
+    for (int k = 0; k < nc; ++k) s[k] = 0.0;
+    if (valid) {
+        if (a.contrib) bonded_slice_sum(a.contrib, a.gptr, a.gidx, i, sl, s[0], s[1], s[2]);
+        if (a.isspa.part) {
+            double t[6] = {0, 0, 0, 0, 0, 0};
+            isspa_slice_sum<6>(a.isspa, i, sl, t);
+            for (int k = 0; k < 6; ++k) s[3 + k] = t[k];
+        }
+        if (a.nb_part && sl == kSumSlices - 1) {
+            double t[3] = {0, 0, 0};
+            nonbond_chunk_sum<3>(a.nb_wins, a.nb_atom_win, a.nb_part, a.nb_B, i, t);
+            for (int k = 0; k < 3; ++k) nbs[k][al] = t[k];
+        }
+    }
+    if (sl > 0)
+        for (int k = 0; k < nc; ++k) red[sl - 1][k][al] = s[k];
+    __syncthreads();
+    if (sl != 0 || !valid) return;
+    for (int t = 0; t < kSumSlices - 1; ++t)
+        for (int k = 0; k < nc; ++k) s[k] += red[t][k][al];
+    float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.contrib) {
+        f.x += (float)s[0];
+        f.y += (float)s[1];
+        f.z += (float)s[2];
+    }
+    if (a.isspa.part) {
+        const float3 fi = isspa_atom_force(a.isspa, i, s + 3);
+        f.x += fi.x;
+        f.y += fi.y;
+        f.z += fi.z;
+    }
+    if (a.nb_part) {
+        f.x += (float)nbs[0][al];
+        f.y += (float)nbs[1][al];
+        f.z += (float)nbs[2][al];
+    }
+    a.f[i] = f;
 }
 
 // ---------------------------------------------------------------------------
@@ -689,11 +713,19 @@ hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t
 hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s)
 {
     if (a.ntot == 0) return hipSuccess;
-    const int nb = (a.ntot + kRedAtoms - 1) / kRedAtoms;
+    const int nb = (a.ntot + kSumAtoms - 1) / kSumAtoms;
     if (split)
         hipLaunchKernelGGL(k_force_reduce<true>, dim3(nb), dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_force_reduce<false>, dim3(nb), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_assemble_forces, dim3((a.n + kSumAtoms - 1) / kSumAtoms), dim3(kBlock), 0,
+                       s, a);
     return hipGetLastError();
 }
 

@@ -133,6 +133,22 @@ hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s);
 hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s);
 hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s);
 
+// One step's force of the MD engine in one launch: f = bonded gather + IS-SPA
+// reduction + all-pairs chunk reduction (each optional: null contrib, null
+// isspa.part, null nb_part), overwriting f (w = 0).
+struct AssembleArgs {
+    int n;
+    float4 *f;
+    const float4 *contrib;  // bonded contribution slots
+    const int *gptr, *gidx;
+    ReduceArgs isspa;       // IS-SPA chunk partials (isspa.f unused)
+    const WinDesc *nb_wins; // all-pairs chunk partials
+    const int *nb_atom_win;
+    const double *nb_part;
+    long long nb_B;
+};
+hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s);
+
 // ---- nonbond ----
 struct NonbondArgs {
     const WinDesc *wins;

@@ -9,6 +9,7 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "pbc_device.hpp"
+#include "reduce_device.hpp"
 
 namespace isspa {
 
@@ -148,22 +149,21 @@ __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restric
 {
     const int i = blockIdx.x * kBlk + threadIdx.x;
     if (i >= ntot) return;
-    const WinDesc W = wins[atom_win[i]];
-    const int il = i - W.a0;
-    double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < W.nbc; ++c) {
-        const long long o = W.npo + (long long)c * W.n + il;
-        for (int k = 0; k < comps; ++k) s[k] += part[k * B + o];
-    }
-    if (comps == 3 && f) {
-        float4 v = f[i];
-        v.x += (float)s[0];
-        v.y += (float)s[1];
-        v.z += (float)s[2];
-        f[i] = v;
-    } else if (comps == 3) {
-        for (int k = 0; k < 3; ++k) o0[3 * i + k] += s[k];
+    if (comps == 3) {
+        double s[3] = {0, 0, 0};
+        nonbond_chunk_sum<3>(wins, atom_win, part, B, i, s);
+        if (f) {
+            float4 v = f[i];
+            v.x += (float)s[0];
+            v.y += (float)s[1];
+            v.z += (float)s[2];
+            f[i] = v;
+        } else {
+            for (int k = 0; k < 3; ++k) o0[3 * i + k] += s[k];
+        }
     } else {
+        double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        nonbond_chunk_sum<9>(wins, atom_win, part, B, i, s);
         for (int k = 0; k < 3; ++k) {
             o0[3 * i + k] = s[k];
             o1[3 * i + k] = s[3 + k];
@@ -355,27 +355,17 @@ hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s)
 // A block owns 32 atoms and 8 slot slices: slice t sums slots t, t + 8, ...
 // of its atom in order, then slice 0 adds slices 1..7 in order (a fixed order
 // independent of scheduling, with 8 times the waves of one lane per atom).
-constexpr int kGatherAtoms = 32;
-constexpr int kGatherSlices = kBlk / kGatherAtoms;
-
 __global__ __launch_bounds__(kBlk) void k_bonded_gather(const float4 *__restrict__ contrib,
                                                         const int *__restrict__ ptr,
                                                         const int *__restrict__ idx, int n,
                                                         float4 *f)
 {
-    __shared__ double red[kGatherSlices - 1][3][kGatherAtoms];
-    const int al = threadIdx.x % kGatherAtoms, sl = threadIdx.x / kGatherAtoms;
-    const int i = blockIdx.x * kGatherAtoms + al;
+    __shared__ double red[kSumSlices - 1][3][kSumAtoms];
+    const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
+    const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < n;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    if (valid) {
-        for (int e = ptr[i] + sl; e < ptr[i + 1]; e += kGatherSlices) {
-            const float4 c = contrib[idx[e]];
-            sx += c.x;
-            sy += c.y;
-            sz += c.z;
-        }
-    }
+    if (valid) bonded_slice_sum(contrib, ptr, idx, i, sl, sx, sy, sz);
     if (sl > 0) {
         red[sl - 1][0][al] = sx;
         red[sl - 1][1][al] = sy;
@@ -383,7 +373,7 @@ __global__ __launch_bounds__(kBlk) void k_bonded_gather(const float4 *__restrict
     }
     __syncthreads();
     if (sl != 0 || !valid) return;
-    for (int t = 0; t < kGatherSlices - 1; ++t) {
+    for (int t = 0; t < kSumSlices - 1; ++t) {
         sx += red[t][0][al];
         sy += red[t][1][al];
         sz += red[t][2][al];
@@ -399,7 +389,7 @@ hipError_t launch_bonded_gather(const float4 *contrib, const int *ptr, const int
                                 float4 *f, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bonded_gather, dim3((n + kGatherAtoms - 1) / kGatherAtoms), dim3(kBlk), 0,
+    hipLaunchKernelGGL(k_bonded_gather, dim3((n + kSumAtoms - 1) / kSumAtoms), dim3(kBlk), 0,
                        s, contrib, ptr, idx, n, f);
     return hipGetLastError();
 }

@@ -1,0 +1,102 @@
+// Fixed-order per-atom sums shared by the reduction kernels: the IS-SPA chunk
+// reduction, the bonded gather, the all-pairs chunk reduction and the MD
+// engine's force assembly, which does all three in one launch.  Each kernel
+// that sums a term calls the same helper, so a term's per-atom sum has the
+// same operations in the same order whichever kernel computes it.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace isspa {
+
+// A 256-thread reduction block owns kSumAtoms atoms and kSumSlices slices of
+// each atom's list: slice t takes entries t, t + S, t + 2S, ... in order, and
+// slice 0 then adds slices 1..S-1 in order.  The slices give the kernel S
+// times the waves of one lane per atom, which small systems need to hide the
+// load latency.
+constexpr int kSumAtoms = 32;
+constexpr int kSumSlices = 8;
+
+// IS-SPA force partials of atom i (COMPS = 6, or 9 in split mode), chunks of
+// slice sl; four chunks' loads in flight at a time, added in chunk order.
+template <int COMPS>
+__device__ inline void isspa_slice_sum(const ReduceArgs &a, int i, int sl, double (&s)[COMPS])
+{
+    const WinDesc W = a.wins[a.atom_win[i]];
+    const long long base = W.ppo + (i - W.a0);
+    int c = sl;
+    for (; c + 3 * kSumSlices < W.chunks; c += 4 * kSumSlices) {
+        double v[4][COMPS];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long o = base + (long long)(c + u * kSumSlices) * W.n;
+#pragma unroll
+            for (int k = 0; k < COMPS; ++k) v[u][k] = a.part[k * a.C + o];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
+    }
+    for (; c < W.chunks; c += kSumSlices) {
+        const long long o = base + (long long)c * W.n;
+#pragma unroll
+        for (int k = 0; k < COMPS; ++k) s[k] += a.part[k * a.C + o];
+    }
+}
+
+// F_i = -q_i vtot_i S_e - vtot_i S_lj from the summed IS-SPA partials
+__device__ inline float3 isspa_atom_force(const ReduceArgs &a, int i, const double s[6])
+{
+    const float4 x = a.xyzq[i];
+    const double vt = (double)a.vtot[a.itype[i]];
+    const double qv = -(double)x.w * vt;
+    return make_float3((float)(qv * s[0] - vt * s[3]), (float)(qv * s[1] - vt * s[4]),
+                       (float)(qv * s[2] - vt * s[5]));
+}
+
+// bonded contribution slots of atom i, slots ptr[i] + sl, + kSumSlices, ...
+__device__ inline void bonded_slice_sum(const float4 *__restrict__ contrib,
+                                        const int *__restrict__ ptr, const int *__restrict__ idx,
+                                        int i, int sl, double &sx, double &sy, double &sz)
+{
+    for (int e = ptr[i] + sl; e < ptr[i + 1]; e += kSumSlices) {
+        const float4 c = contrib[idx[e]];
+        sx += c.x;
+        sy += c.y;
+        sz += c.z;
+    }
+}
+
+// all-pairs partials of atom i (comps = 3, or 9 in split mode) over every
+// partner chunk of its window in chunk order, four chunks' loads in flight
+template <int COMPS>
+__device__ inline void nonbond_chunk_sum(const WinDesc *__restrict__ wins,
+                                         const int *__restrict__ atom_win,
+                                         const double *__restrict__ part, long long B, int i,
+                                         double (&s)[COMPS])
+{
+    const WinDesc W = wins[atom_win[i]];
+    const long long base = W.npo + (i - W.a0);
+    int c = 0;
+    for (; c + 3 < W.nbc; c += 4) {
+        double v[4][COMPS];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const long long o = base + (long long)(c + u) * W.n;
+#pragma unroll
+            for (int k = 0; k < COMPS; ++k) v[u][k] = part[k * B + o];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
+    }
+    for (; c < W.nbc; ++c) {
+        const long long o = base + (long long)c * W.n;
+#pragma unroll
+        for (int k = 0; k < COMPS; ++k) s[k] += part[k * B + o];
+    }
+}
+
+}  // namespace isspa
