@@ -626,14 +626,21 @@ extern "C" int nonbond_model_destroy(nonbond_model *m)
     return ISSPA_OK;
 }
 
-int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s)
+int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s,
+                     const int *pbc_flag)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
-    HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
+    // A flag set with a larger margin only sends a window through the
+    // minimum-image loop when no image is needed, where that loop's shift
+    // (fma(-L, 0, r)) returns r unchanged: the result is the same bit for bit.
+    if (!pbc_flag) {
+        HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
+        pbc_flag = m->pbc_flag;
+    }
     NonbondArgs a{};
     a.wins = m->wins; a.blocks = m->blk; a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T;
     a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
-    a.pbc_flag = m->pbc_flag; a.part = m->part; a.B = m->B;
+    a.pbc_flag = pbc_flag; a.part = m->part; a.B = m->B;
     HIPCHK(launch_nonbond(a, m->nblk, split, s));
     return ISSPA_OK;
 }
@@ -1092,8 +1099,8 @@ static int engine_forces(md_engine *e, hipStream_t s)
     }
     if (fork)
         HIPCHK(hipStreamWaitEvent(s, e->ev_join, 0));
-    else if (e->nb)
-        CHK(nonbond_partials(e->nb, e->x, false, s));
+    else if (e->nb)  // after the IS-SPA points launch, whose flags it reuses
+        CHK(nonbond_partials(e->nb, e->x, false, s, e->isspa ? e->isspa->pbc_flag : nullptr));
     if (e->nb) {
         a.nb_wins = e->nb->wins;
         a.nb_atom_win = e->nb->atom_win;

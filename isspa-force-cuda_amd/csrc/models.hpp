@@ -86,6 +86,10 @@ ForceArgs force_args(isspa_model *m, const float4 *x);
 int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,
                 double *o1, double *o2, hipStream_t s);
 // the same in two halves: per-chunk partials, then their fixed-order sum
-int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s);
+// pbc_flag: per-window minimum-image flags already computed on stream s with a
+// margin at least as large as the all-pairs one (the IS-SPA model's), or null
+// to compute them here
+int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s,
+                     const int *pbc_flag = nullptr);
 int nonbond_reduce(nonbond_model *m, bool split, float4 *f, double *o0, double *o1, double *o2,
                    hipStream_t s);

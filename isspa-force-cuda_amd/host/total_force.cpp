@@ -92,6 +92,7 @@ int main(int argc, char *argv[])
     md_engine *eng = nullptr;
     bonded_model *all_bonded = nullptr;
     void *stream = nullptr;
+    int steps_per_graph = 1;
     if (graph) {
         // one combined bonded model and the lazily built term models
         std::vector<float> A(atoms.nTypes * (atoms.nTypes + 1) / 2), B(A.size());
@@ -118,10 +119,17 @@ int main(int argc, char *argv[])
                                      configs.lbox, (uint64_t)(uint32_t)configs.seed, 0u, &eng),
                     "md_engine_create");
         isspa_check(isspa_stream_create(&stream), "isspa_stream_create");
-        isspa_check(md_engine_capture(eng, 1, stream), "md_engine_capture");
+        // several steps per graph replay when they divide the write interval,
+        // so every write still falls on a replay boundary
+        for (int k : {10, 5, 2})
+            if (configs.deltaWrite % k == 0 && configs.nSteps >= k) {
+                steps_per_graph = k;
+                break;
+            }
+        isspa_check(md_engine_capture(eng, steps_per_graph, stream), "md_engine_capture");
     }
 
-    for (int step = 0; step < configs.nSteps; ++step) {
+    for (int step = 0; step < configs.nSteps;) {
         const bool write = step % configs.deltaWrite == 0;
         if (graph) {
             if (write) {
@@ -138,7 +146,13 @@ int main(int argc, char *argv[])
                 if (configs.us == 1) bias.print_cv(step, configs.lbox);
                 times.stopWriteTimer();
             }
+            if (configs.nSteps - step < steps_per_graph) {
+                // the last steps, fewer than one replay: a one-step graph
+                steps_per_graph = 1;
+                isspa_check(md_engine_capture(eng, 1, stream), "md_engine_capture");
+            }
             isspa_check(md_engine_replay(eng, 1, stream), "md_engine_replay");
+            step += steps_per_graph;
             continue;
         }
         isspa_check(isspa_zero(atoms.for_d, fbytes, nullptr), "isspa_zero");
@@ -160,6 +174,7 @@ int main(int argc, char *argv[])
             times.stopWriteTimer();
         }
         times.leapFrogTime += leapfrog_cuda(atoms, configs);
+        ++step;
     }
     if (graph) isspa_check(isspa_stream_sync(stream), "sync");
 

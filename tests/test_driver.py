@@ -117,3 +117,22 @@ def test_driver_umbrella_cv_in_eager_and_graph_modes(tmp_path):
     assert np.array_equal(e[:, 0], g[:, 0])
     assert abs(e[0, 1] - g[0, 1]) < 1e-3  # same input coordinates
     assert np.abs(e[:, 1] - g[:, 1]).max() < 0.05  # FP32 summation order only
+
+
+@pytest.mark.gpu
+def test_driver_graph_steps_per_replay_leave_the_trajectory_unchanged(tmp_path):
+    """Graph mode replays 10 steps per graph when they divide the write
+    interval, and the last steps (23 = 2 x 10 + 3) with a one-step graph.  The
+    trajectory equals, digit for digit, the run that replays one step per
+    graph (write interval 23: no divisor among 10, 5, 2)."""
+    out = {}
+    for name, delta in (("ten", 10), ("one", 23)):
+        d = tmp_path / name
+        d.mkdir()
+        cfg = _deck(d, n_steps=23, delta=delta)
+        r = subprocess.run([EXE, str(cfg), "--graph"], capture_output=True, text=True, cwd=d,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        out[name] = ((d / "pos.rst").read_text(), (d / "vel.rst").read_text())
+    assert out["ten"][0] == out["one"][0]
+    assert out["ten"][1] == out["one"][1]
