@@ -180,6 +180,48 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
     return out
 
 
+ADI2_GTX1080_NS_PER_DAY = 1270.51  # timing_tests/ADI2_US/window_6.5/ADI2.run0.log:66
+
+
+def adi2_bench(steps, warmup, per_graph):
+    """The reference's own published timing run: ADI2 (44 atoms, two 22-atom
+    chains), nMC = 100, box 200 A, one centre-of-mass distance restraint
+    (atoms 1-22 against 23-44, k = 20, x0 = 6.5 A), 298 K, pnu = 0.01,
+    dt = 2 fs (timing_tests/ADI2_US/window_6.5/ADI2.run0.log).  The whole MD
+    step runs on the engine, replayed from a graph of per_graph steps."""
+    import torch
+    from isspa_amd import capi, io
+    from isspa_amd.engine import GpuSystem
+    from isspa_amd.system import UmbrellaRestraint
+    d = os.path.join(ROOT, "tests", "golden", "adi2")
+    s = io.load_system(os.path.join(d, "ADI2.prmtop"), os.path.join(d, "ADI2.isspatop"),
+                       os.path.join(d, "ADI2.rst7"), lbox=200.0, n_mc=100, pnu=0.01,
+                       seed=34669, name="ADI2")
+    s.us = UmbrellaRestraint.from_groups(range(0, 22), range(22, 44), s.true_mass, 20.0, 6.5)
+    g = GpuSystem(s, "cuda:0")
+    g.set_step(0)
+    g.capture(per_graph)
+    eng = g.make_engine()
+    st = torch.cuda.Stream()
+    n = max(1, steps // per_graph)
+    with torch.cuda.stream(st):
+        # the library's own replay loop: one graph launch per per_graph steps
+        _time_replays(lambda: capi.check(capi.lib.md_engine_replay(
+            eng, max(1, warmup // per_graph), st.cuda_stream)), 1)
+        wall = _time_replays(lambda: capi.check(capi.lib.md_engine_replay(
+            eng, n, st.cuda_stream)), 1)
+    g.close()
+    sec = wall / (n * per_graph)
+    log(f"ADI2: {sec * 1e6:.2f} us per step ({per_graph} steps per graph), "
+        f"{0.1728 / sec:.0f} ns/day")
+    return {"workload": "ADI2 44 atoms, nMC=100, box 200 A, one distance restraint (k=20, x0=6.5 A); "
+                        "whole MD step on the engine, graph-replayed",
+            "steps": n * per_graph, "steps_per_graph": per_graph,
+            "us_per_step": round(sec * 1e6, 3), "ns_per_day": round(0.1728 / sec, 1),
+            "reference_gtx1080_ns_per_day": ADI2_GTX1080_NS_PER_DAY,
+            "ratio_vs_reference": round(0.1728 / sec / ADI2_GTX1080_NS_PER_DAY, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,7 +240,12 @@ def main():
     ap.add_argument("--plain-rounds", type=int, default=100,
                     help="rounds of the per-window loop in the batched comparison (0: skip it)")
     ap.add_argument("--steps-per-graph", type=int, default=1,
-                    help="steps captured per graph in the batched comparison (both sides)")
+                    help="steps captured per graph in the batched comparison (both sides) "
+                         "and in --adi2-only")
+    ap.add_argument("--adi2-only", action="store_true",
+                    help="run only the reference's published ADI2 configuration on the engine "
+                         "and print its JSON object")
+    ap.add_argument("--adi2-steps", type=int, default=20000)
     ap.add_argument("--tiling", default="", help="explicit field splits,force chunks (no tuning)")
     ap.add_argument("--target-waves", type=int, default=0, help="automatic tiling at this target")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -209,6 +256,10 @@ def main():
               "(the per-GPU shard design is documented, no multi-GPU path runs here)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.adi2_only:
+        print(json.dumps({"adi2": adi2_bench(args.adi2_steps, 200, args.steps_per_graph)}))
+        return
 
     if args.batch_only:
         print(json.dumps({"batched": batched_bench(args.batch_steps, args.plain_rounds,

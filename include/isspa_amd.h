@@ -143,8 +143,10 @@ int isspa_model_force_keep(isspa_model *m, const float *xyzq4_d, float *f4_d, fl
 int isspa_model_force_components(isspa_model *m, const float *xyzq4_d, double *f_lj_d,
                                  double *f_pol_d, double *f_cdd_d, uint32_t step, void *stream);
 /* Internal per-step buffers for inspection, copied to device buffers:
- * pts4_d float4[P] (.w = owner rmax^2), g_d double[P], cnt_d int[P],
- * E_d double[P*3], E0_d double[P*3].  Any pointer may be NULL. */
+ * pts4_d float4[P] (.w = owner rmax^2) from the model's last force call of any
+ * kind; g_d double[P], cnt_d int[P], E_d double[P*3], E0_d double[P*3] from
+ * its last isspa_model_force_components call (the force paths skip writing
+ * them).  Any pointer may be NULL. */
 int isspa_model_get_field(isspa_model *m, float *pts4_d, double *g_d, int *cnt_d, double *E_d,
                           double *E0_d, void *stream);
 /* Stage timing for tuning and roofline accounting (no reference counterpart;

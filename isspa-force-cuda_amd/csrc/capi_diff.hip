@@ -186,7 +186,8 @@ static int diff_isspa_forward(isspa_diff *m, const double *q_d, double *f_d, hip
     FieldArgs fa = field_args(im, m->xq);
     fa.dtypes = m->dtypes;
     HIPCHK(launch_field(fa, im->nfblk, s));
-    HIPCHK(launch_epilogue(epilogue_args(im), s));
+    // the adjoint reads the per-point g, count and E
+    HIPCHK(launch_epilogue(epilogue_args(im, true), s));
     ForceArgs ka = force_args(im, m->xq);
     ka.dtypes = m->dtypes;
     HIPCHK(launch_force(ka, im->npblk, false, s));

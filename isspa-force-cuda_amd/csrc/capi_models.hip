@@ -315,15 +315,21 @@ FieldArgs field_args(isspa_model *m, const float4 *x)
     return fa;
 }
 
-EpilogueArgs epilogue_args(isspa_model *m)
+// inspect: also write the per-point g, count, E and E0 that
+// isspa_model_get_field returns and the adjoint reads (the component entry
+// point and the differentiable forward)
+EpilogueArgs epilogue_args(isspa_model *m, bool inspect)
 {
     EpilogueArgs ea{};
-    ea.wins = m->wins; ea.atom_win = m->atom_win; ea.pts = m->pts; ea.Ptot = m->P;
-    ea.nmc = m->d.n_mc; ea.part_g = m->part_g; ea.part_cnt = m->part_cnt;
+    ea.wins = m->wins; ea.nwin = m->lay.W; ea.atom_win = m->atom_win; ea.pts = m->pts;
+    ea.Ptot = m->P; ea.nmc = m->d.n_mc; ea.part_g = m->part_g; ea.part_cnt = m->part_cnt;
     ea.part_s0 = m->part_s0; ea.part_ein = m->part_ein; ea.F = m->F;
     ea.e0 = m->d.e0; ea.p0 = m->d.p0; ea.q0 = m->d.q0; ea.o0 = m->d.o0;
-    ea.rec = m->rec; ea.aux = rec_aux(m); ea.g_out = m->g_out; ea.cnt_out = m->cnt_out; ea.E_out = m->E_out;
-    ea.E0_out = m->E0_out;
+    ea.rec = m->rec; ea.aux = rec_aux(m);
+    if (inspect) {
+        ea.g_out = m->g_out; ea.cnt_out = m->cnt_out; ea.E_out = m->E_out;
+        ea.E0_out = m->E0_out;
+    }
     return ea;
 }
 
@@ -349,14 +355,14 @@ static ReduceArgs reduce_args(isspa_model *m, const float4 *x, float4 *f, double
 
 // points + field + epilogue (shared by the total and component entry points)
 static int isspa_field_pass(isspa_model *m, const float *xyzq4_d, uint32_t step,
-                            const uint32_t *step_d, hipStream_t s)
+                            const uint32_t *step_d, hipStream_t s, bool inspect = false)
 {
     const float4 *x = reinterpret_cast<const float4 *>(xyzq4_d);
     HIPCHK(launch_mc_points(m->wins, m->atom_win, x, m->itype, m->rmax, m->rmax2, m->d.n_atoms,
                             m->d.n_mc, m->d.seed, step, step_d, m->pts, s, m->lay.W, m->rmax_all,
                             m->pbc_flag));
     HIPCHK(launch_field(field_args(m, x), m->nfblk, s));
-    HIPCHK(launch_epilogue(epilogue_args(m), s));
+    HIPCHK(launch_epilogue(epilogue_args(m, inspect), s));
     return ISSPA_OK;
 }
 
@@ -398,7 +404,7 @@ extern "C" int isspa_model_force_components(isspa_model *m, const float *xyzq4_d
 {
     if (!m || !xyzq4_d || !f_lj_d || !f_pol_d || !f_cdd_d)
         return fail(ISSPA_ERR_ARG, "isspa_model_force_components: null argument");
-    CHK(isspa_field_pass(m, xyzq4_d, step, nullptr, S(stream)));
+    CHK(isspa_field_pass(m, xyzq4_d, step, nullptr, S(stream), true));
     return isspa_force_pass(m, xyzq4_d, true, nullptr, f_lj_d, f_pol_d, f_cdd_d, S(stream));
 }
 
@@ -425,7 +431,7 @@ extern "C" int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int rep
                                      m->d.n_atoms, m->d.n_mc, m->d.seed, 0u, nullptr, m->pts, s);
                 break;
             case 1: e = launch_field(field_args(m, x), m->nfblk, s); break;
-            case 2: e = launch_epilogue(epilogue_args(m), s); break;
+            case 2: e = launch_epilogue(epilogue_args(m, false), s); break;
             case 3: e = launch_force(force_args(m, x), m->npblk, false, s); break;
             default:
                 e = launch_force_reduce(reduce_args(m, x, m->prof_f, nullptr, nullptr, nullptr),

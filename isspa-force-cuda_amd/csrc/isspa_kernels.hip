@@ -269,37 +269,46 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
 // Below x = 5: <P_l> = i_l(x)/i_0(x) from the all-positive power series
 // i_l(x) = x^l sum_k (x^2/2)^k / (k! (2l+2k+1)!!), 30 terms; above 5 the
 // upward recurrence from coth, which then cancels by at most ~3x.
-__device__ __forceinline__ void langevin_moments(double x, double &m1, double &m2, double &m3)
+__device__ __forceinline__ void langevin_moments(double x, double inv_x, double &m1, double &m2,
+                                                 double &m3)
 {
     if (x < 5.0) {
+        // all 30 terms of the four series, advanced together as four
+        // independent FP64 chains with compile-time coefficients: past the
+        // first few terms they add nothing (below 1e-17 of the sum for small
+        // x), but a fixed count keeps the loop free of branches and loads
         const double h = 0.5 * x * x;
-        double s[4];
+        double t[4], s[4];
 #pragma unroll
         for (int l = 0; l < 4; ++l) {
-            // terms fall below 1e-17 of the sum well before k = 30 for small x
-            double t = kLangevin.first[l], sum = t;
-            for (int k = 0; k < kLgTerms && t > 1e-17 * sum; ++k) {
-                t *= h * kLangevin.step[l][k];
-                sum += t;
-            }
-            s[l] = sum;
+            t[l] = kLangevin.first[l];
+            s[l] = t[l];
         }
-        m1 = x * s[1] / s[0];
-        m2 = x * x * s[2] / s[0];
-        m3 = x * x * x * s[3] / s[0];
+#pragma unroll
+        for (int k = 0; k < kLgTerms; ++k)
+#pragma unroll
+            for (int l = 0; l < 4; ++l) {
+                t[l] *= h * kLangevin.step[l][k];
+                s[l] += t[l];
+            }
+        const double r = 1.0 / s[0];
+        m1 = x * s[1] * r;
+        m2 = x * x * s[2] * r;
+        m3 = x * x * x * s[3] * r;
         return;
     }
     const double coth = 1.0 / tanh(x);
-    m1 = coth - 1.0 / x;
-    m2 = 1.0 - 3.0 * m1 / x;
-    m3 = m1 - 5.0 * m2 / x;
+    m1 = coth - inv_x;
+    m2 = 1.0 - 3.0 * m1 * inv_x;
+    m3 = m1 - 5.0 * m2 * inv_x;
 }
 
 __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
 {
     const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
     if (gp >= a.Ptot) return;
-    const WinDesc W = a.wins[a.atom_win[gp / a.nmc]];
+    // one window: no per-point window lookup
+    const WinDesc W = a.nwin == 1 ? a.wins[0] : a.wins[a.atom_win[gp / a.nmc]];
     const int P = W.n * a.nmc;
     const long long m = gp - (long long)W.a0 * a.nmc;
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
@@ -319,10 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
         E[c] = E0[c] + ei[c];
     }
     const double x = sqrt(E[0] * E[0] + E[1] * E[1] + E[2] * E[2]);
+    const double inv_x = x > 0.0 ? 1.0 / x : 0.0;
     double eh[3];
-    for (int c = 0; c < 3; ++c) eh[c] = x > 0.0 ? E[c] / x : 0.0;
+    for (int c = 0; c < 3; ++c) eh[c] = E[c] * inv_x;
     double m1, m2, m3;
-    langevin_moments(x, m1, m2, m3);
+    langevin_moments(x, inv_x, m1, m2, m3);
     // cnt >= 1 always: the owner atom lies inside its own sphere
     const double inv_cnt = 1.0 / (double)(cnt > 0 ? cnt : 1);
     const double w = g * inv_cnt;
@@ -330,12 +340,14 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     const double A1 = (double)a.p0 * m1 * w;
     const double A2 = (double)a.q0 * m2 * w;
     const double A3 = (double)a.o0 * m3 * w;
+    double c3[3];  // cp E0 / 3
+    for (int c = 0; c < 3; ++c) c3[c] = cp * E0[c] * (1.0 / 3.0);
     const float4 p = a.pts[gp];
     float4 *r4 = reinterpret_cast<float4 *>(a.rec + gp * kRecFloats);
     r4[0] = make_float4(p.x, p.y, p.z, (float)eh[0]);
-    r4[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + cp * E0[0] / 3.0),
-                        (float)(A1 * eh[1] + cp * E0[1] / 3.0));
-    r4[2] = make_float4((float)(A1 * eh[2] + cp * E0[2] / 3.0),
+    r4[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + c3[0]),
+                        (float)(A1 * eh[1] + c3[1]));
+    r4[2] = make_float4((float)(A1 * eh[2] + c3[2]),
                         (float)(3.0 * A1 * eh[0] + cp * E0[0]),
                         (float)(3.0 * A1 * eh[1] + cp * E0[1]),
                         (float)(3.0 * A1 * eh[2] + cp * E0[2]));

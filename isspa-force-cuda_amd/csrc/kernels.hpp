@@ -76,6 +76,7 @@ struct FieldArgs {
 
 struct EpilogueArgs {
     const WinDesc *wins;
+    int nwin;
     const int *atom_win;
     const float4 *pts;
     long long Ptot;
@@ -87,7 +88,7 @@ struct EpilogueArgs {
     float e0, p0, q0, o0;
     float *rec;         // [P][16]
     float4 *aux;        // [P] {cp E0, A1}
-    double *g_out;      // [P]      (may be null)
+    double *g_out;      // [P]      inspection outputs, each may be null
     int *cnt_out;       // [P]
     double *E_out;      // [P][3]
     double *E0_out;     // [P][3]

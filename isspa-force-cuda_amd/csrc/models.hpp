@@ -79,7 +79,7 @@ inline float4 *rec_aux(isspa_model *m)
 }
 
 FieldArgs field_args(isspa_model *m, const float4 *x);
-EpilogueArgs epilogue_args(isspa_model *m);
+EpilogueArgs epilogue_args(isspa_model *m, bool inspect);
 ForceArgs force_args(isspa_model *m, const float4 *x);
 // the all-pairs term for positions x: split -> three FP64 component outputs;
 // otherwise f (float4, +=) or, with f null, o0 (double[N*3], +=)
