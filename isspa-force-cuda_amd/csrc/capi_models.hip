@@ -980,7 +980,7 @@ struct md_engine {
     uint32_t sys = 0;
     WinDesc *wins = nullptr;   // integrator windows (owned)
     int *atom_win = nullptr;
-    uint32_t *step_d = nullptr;
+    uint32_t *step_d = nullptr;  // [2]: step, leap-frog blocks done
     double *com_d = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1035,10 +1035,11 @@ extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_mo
     for (int w = 0; rc == ISSPA_OK && w < lay.W; ++w) wd.push_back(base_desc(lay, w));
     if (rc == ISSPA_OK) rc = upload(&e->wins, wd.data(), wd.size());
     if (rc == ISSPA_OK) rc = upload(&e->atom_win, lay.atom_win.data(), lay.atom_win.size());
-    if (rc == ISSPA_OK) rc = dalloc(&e->step_d, 1);
+    // {step, leap-frog blocks done}: the leap-frog launch advances the step
+    if (rc == ISSPA_OK) rc = dalloc(&e->step_d, 2);
     if (rc == ISSPA_OK) rc = dalloc(&e->com_d, 6 * (size_t)(us ? us->nres : 1));
     if (rc == ISSPA_OK) {
-        hipError_t he = hipMemset(e->step_d, 0, sizeof(uint32_t));
+        hipError_t he = hipMemset(e->step_d, 0, 2 * sizeof(uint32_t));
         if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
     }
     // below a few thousand atoms the all-pairs kernel is a few microseconds and
@@ -1123,8 +1124,8 @@ static int engine_step(md_engine *e, hipStream_t s)
     CHK(engine_forces(e, s));
     HIPCHK(launch_leapfrog(e->wins, e->atom_win, WinDesc{}, e->n, reinterpret_cast<float4 *>(e->x),
                            reinterpret_cast<float4 *>(e->v), reinterpret_cast<const float4 *>(e->f),
-                           e->kT, e->dt, andersen_threshold(e->pnu), e->seed, 0u, e->step_d, s));
-    HIPCHK(launch_step_advance(e->step_d, s));
+                           e->kT, e->dt, andersen_threshold(e->pnu), e->seed, 0u, e->step_d, s,
+                           e->step_d));
     return ISSPA_OK;
 }
 

@@ -293,7 +293,7 @@ hipError_t launch_type_reduce(const double *gatom, const int *itype, int n, int 
 hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDesc &one,
                            int ntot, float4 *xyzq, float4 *vel, const float4 *f, float kT,
                            float dt, uint32_t thr, uint64_t seed, uint32_t step,
-                           const uint32_t *step_d, hipStream_t s);
+                           const uint32_t *step_d, hipStream_t s, uint32_t *advance = nullptr);
 // one workgroup per restraint; bias entries [boff[r], boff[r+1]) {atom, group}
 hipError_t launch_us(int nres, const int *boff, const int2 *atoms, const float *mass,
                      const float2 *kx /* {k, x0} */, const float *L, const float4 *xyzq,

@@ -427,18 +427,13 @@ hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, dou
 // velocity is redrawn from Maxwell-Boltzmann, then half-kicked (quirk Q21).
 // Atom i of window w draws with counter {i - a0_w, draw, step, sys_w}.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlk) void k_leapfrog(const WinDesc *__restrict__ wins,
-                                                   const int *__restrict__ atom_win, WinDesc one,
-                                                   int ntot,
-                                                   float4 *xyzq, float4 *vel,
-                                                   const float4 *__restrict__ f, float kT,
-                                                   float dt, uint32_t thr, uint64_t seed,
-                                                   uint32_t step_arg,
-                                                   const uint32_t *__restrict__ step_d)
+__device__ __forceinline__ void leapfrog_atom(const WinDesc *__restrict__ wins,
+                                              const int *__restrict__ atom_win,
+                                              const WinDesc &one, int i, float4 *xyzq,
+                                              float4 *vel, const float4 *__restrict__ f,
+                                              float kT, float dt, uint32_t thr, uint64_t seed,
+                                              uint32_t step)
 {
-    const int i = blockIdx.x * kBlk + threadIdx.x;
-    if (i >= ntot) return;
-    const uint32_t step = step_d ? *step_d : step_arg;
     const WinDesc W = atom_win ? wins[atom_win[i]] : one;
     const uint32_t il = (uint32_t)(i - W.a0);
     const uint4 o = philox_draw(seed, il, kStreamAndersen, 0u, step, W.sys);
@@ -476,14 +471,42 @@ __global__ __launch_bounds__(kBlk) void k_leapfrog(const WinDesc *__restrict__ w
     xyzq[i] = x;
 }
 
+__global__ __launch_bounds__(kBlk) void k_leapfrog(const WinDesc *__restrict__ wins,
+                                                   const int *__restrict__ atom_win, WinDesc one,
+                                                   int ntot,
+                                                   float4 *xyzq, float4 *vel,
+                                                   const float4 *__restrict__ f, float kT,
+                                                   float dt, uint32_t thr, uint64_t seed,
+                                                   uint32_t step_arg,
+                                                   const uint32_t *step_d, uint32_t *advance)
+{
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    const uint32_t step = step_d ? *step_d : step_arg;
+    if (i < ntot) leapfrog_atom(wins, atom_win, one, i, xyzq, vel, f, kT, dt, thr, seed, step);
+    if (!advance) return;
+    // advance = {step, blocks done}: the last block to finish moves the step
+    // on and clears the count.  Every block read the step before its count,
+    // so no block of this launch sees the new value.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(advance + 1, 1u);
+        if (done == gridDim.x - 1) {
+            advance[0] = step + 1u;
+            atomicExch(advance + 1, 0u);
+            __threadfence();
+        }
+    }
+}
+
 hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDesc &one,
                            int ntot, float4 *xyzq, float4 *vel, const float4 *f, float kT,
                            float dt, uint32_t thr, uint64_t seed, uint32_t step,
-                           const uint32_t *step_d, hipStream_t s)
+                           const uint32_t *step_d, hipStream_t s, uint32_t *advance)
 {
-    if (ntot == 0) return hipSuccess;
+    if (ntot == 0) return advance ? launch_step_advance(advance, s) : hipSuccess;
     hipLaunchKernelGGL(k_leapfrog, dim3(nblocks(ntot)), dim3(kBlk), 0, s, wins, atom_win, one, ntot,
-                       xyzq, vel, f, kT, dt, thr, seed, step, step_d);
+                       xyzq, vel, f, kT, dt, thr, seed, step, step_d, advance);
     return hipGetLastError();
 }
 

@@ -140,6 +140,31 @@ def test_engine_forces_and_step(cfg0):
     g.close()
 
 
+@pytest.mark.parametrize("mode", ["eager", "graph"])
+def test_engine_step_counter_advances_once_per_step(cfg1, mode):
+    """The engine keeps its step on the device and the leap-frog launch moves
+    it on.  After set_step(5) and 4 steps (eager, or two replays of a 2-step
+    graph) the next force evaluation draws the MC points of step 9 around the
+    current coordinates, bit for bit as the oracle draws them."""
+    s = cfg1
+    g = _gpu(s)
+    g.set_step(5)
+    if mode == "eager":
+        g.step(4)
+    else:
+        g.capture(2)
+        g.replay(2)
+    g.forces()
+    torch.cuda.synchronize()
+    pts = g.field()["pts"].cpu().numpy()
+    x = g.x.cpu().numpy()
+    g.close()
+    ref = po.mc_points(s, xyzq=x, step=9)
+    assert np.array_equal(pts[:, :3].view(np.uint32), ref[:, :3].view(np.uint32))
+    wrong = po.mc_points(s, xyzq=x, step=8)
+    assert not np.array_equal(pts[:, :3].view(np.uint32), wrong[:, :3].view(np.uint32))
+
+
 def test_graph_replay_matches_eager(cfg1):
     """Three replays of a captured 2-step graph, with eager work and
     synchronisations between them, against the same steps run eagerly."""
