@@ -33,3 +33,56 @@ def test_round_aligned_candidates():
         assert v == 206 or autotune.round_fill(pb, v) >= 0.85
     assert 9 in s and 13 in s          # 6.87 and 9.92 rounds
     assert len(s) <= 7 and len(c) <= 7
+
+
+class _FakeModel:
+    """Stands in for GpuSystem: stage times are a fixed function of the
+    tiling, so the tuner's choice can be checked without a GPU."""
+
+    class _Sys:
+        n_atoms, n_points = 1000, 20000
+
+    windows = None
+    system = _Sys()
+
+    def __init__(self):
+        self.t = (0, 0)
+        self.waves = 32768
+
+    def set_tiling(self, s, c):
+        self.t = (s, c)
+
+    def set_target_waves(self, w):
+        self.waves = w
+
+    def tiling(self):
+        if self.t != (0, 0):
+            return self.t
+        return (max(1, self.waves // 4096), max(1, self.waves // 512))
+
+    def profile(self, reps):
+        s, c = self.tiling()
+        # the field stages prefer 8 splits, the force stages 64 chunks
+        return {"points": 0.0, "field": 1.0 + abs(s - 8) * 0.01, "epilogue": 0.1,
+                "force": 2.0 + abs(c - 64) * 0.001, "reduce": 0.1}
+
+
+def test_tune_picks_fastest_pass_and_confirm_overrides():
+    g = _FakeModel()
+    r = autotune.tune(g, reps=1)
+    assert r.measure == "IS-SPA pass"
+    assert g.tiling() == (8, 64) and abs(r.ms - 3.2) < 1e-9
+    # the leaders are waves 32768 and tiling (8, 64) (both 3.2 ms) and tiling
+    # (8, 32) (3.232 ms); a confirmation that prefers (8, 32) decides
+    calls = []
+
+    def confirm(model, reps):
+        calls.append(model.tiling())
+        return 1.0 if model.tiling() == (8, 32) else 5.0
+
+    g2 = _FakeModel()
+    r2 = autotune.tune(g2, reps=1, confirm=confirm)
+    assert r2.measure == "engine forces"
+    assert sorted(calls) == [(8, 32), (8, 64), (8, 64)]
+    assert r2.kind == "tiling" and r2.value == (8, 32)
+    assert g2.tiling() == (8, 32) and r2.ms == 1.0
