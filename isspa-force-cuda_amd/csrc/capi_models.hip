@@ -816,8 +816,7 @@ extern "C" int bonded_model_destroy(bonded_model *m)
     return ISSPA_OK;
 }
 
-// the per-term kernel: every term's forces into its contribution slots
-static int bonded_terms(bonded_model *m, int terms, const float *xyzq4_d, hipStream_t s)
+static BondedArgs bonded_args(bonded_model *m, int terms, const float *xyzq4_d)
 {
     BondedArgs a{};
     a.xyzq = reinterpret_cast<const float4 *>(xyzq4_d);
@@ -828,7 +827,13 @@ static int bonded_terms(bonded_model *m, int terms, const float *xyzq4_d, hipStr
     a.dih_atoms = m->dih_atoms; a.dih_types = m->dih_types; a.dih_params = m->dih_params;
     a.sc14 = m->sc14; a.ityp = m->ityp; a.lj = m->lj; a.T = m->T;
     a.contrib = m->contrib; a.energy = m->energy; a.terms = terms;
-    HIPCHK(launch_bonded_terms(a, s));
+    return a;
+}
+
+// the per-term kernel: every term's forces into its contribution slots
+static int bonded_terms(bonded_model *m, int terms, const float *xyzq4_d, hipStream_t s)
+{
+    HIPCHK(launch_bonded_terms(bonded_args(m, terms, xyzq4_d), s));
     return ISSPA_OK;
 }
 
@@ -982,6 +987,7 @@ struct md_engine {
     int *atom_win = nullptr;
     uint32_t *step_d = nullptr;  // [2]: step, leap-frog blocks done
     double *com_d = nullptr;
+    float4 *fus = nullptr;       // restraint bias forces of the step (w = 1: restrained)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     unsigned captured_generation = 0;
@@ -1005,7 +1011,7 @@ static void engine_free(md_engine *e)
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     e->side = nullptr;
     e->ev_fork = e->ev_join = nullptr;
-    dfree(e->step_d); dfree(e->com_d); dfree(e->wins); dfree(e->atom_win);
+    dfree(e->step_d); dfree(e->com_d); dfree(e->fus); dfree(e->wins); dfree(e->atom_win);
 }
 
 extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_model *bonded,
@@ -1038,6 +1044,14 @@ extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_mo
     // {step, leap-frog blocks done}: the leap-frog launch advances the step
     if (rc == ISSPA_OK) rc = dalloc(&e->step_d, 2);
     if (rc == ISSPA_OK) rc = dalloc(&e->com_d, 6 * (size_t)(us ? us->nres : 1));
+    if (rc == ISSPA_OK && us) {
+        // entries of atoms no restraint holds stay zero (w = 0) for good
+        rc = dalloc(&e->fus, (size_t)n);
+        if (rc == ISSPA_OK) {
+            hipError_t he = hipMemset(e->fus, 0, (size_t)n * sizeof(float4));
+            if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
+        }
+    }
     if (rc == ISSPA_OK) {
         hipError_t he = hipMemset(e->step_d, 0, 2 * sizeof(uint32_t));
         if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
@@ -1074,10 +1088,12 @@ extern "C" int md_engine_set_step(md_engine *e, uint32_t step, void *stream)
     return ISSPA_OK;
 }
 
-// Every term writes its own partials or contribution slots; one assembly
-// launch then forms f = bonded + IS-SPA + all-pairs (no zeroing launch), bit
-// for bit what zero + bonded_force + isspa_model_force + nonbond_force give,
-// and the restraint adds its force last.
+// One prologue launch computes everything that depends only on the
+// coordinates (IS-SPA minimum-image flags, bonded terms, restraint forces and
+// centres of mass, MC points); the IS-SPA passes and the all-pairs partials
+// follow, and one assembly launch forms f = bonded + IS-SPA + all-pairs +
+// restraint -- bit for bit what zeroing f and calling bonded_force,
+// isspa_model_force, nonbond_force and us_force in that order give.
 static int engine_forces(md_engine *e, hipStream_t s)
 {
     const float4 *x4 = reinterpret_cast<const float4 *>(e->x);
@@ -1090,24 +1106,46 @@ static int engine_forces(md_engine *e, hipStream_t s)
         CHK(nonbond_partials(e->nb, e->x, false, e->side));
         HIPCHK(hipEventRecord(e->ev_join, e->side));
     }
+    PrologueArgs p{};
     AssembleArgs a{};
     a.n = e->n;
     a.f = reinterpret_cast<float4 *>(e->f);
+    isspa_model *m = e->isspa;
+    if (m) {
+        p.nflag = m->lay.W;
+        p.rmax_all = m->rmax_all;
+        p.flag = m->pbc_flag;
+        p.pts = PointsArgs{m->wins, m->atom_win, x4, m->itype, m->rmax, m->rmax2, m->d.n_atoms,
+                           m->d.n_mc, m->d.seed, 0u, e->step_d, m->pts};
+    }
     if (e->bonded) {
-        CHK(bonded_terms(e->bonded, ISSPA_TERM_ALL, e->x, s));
+        p.bonded = bonded_args(e->bonded, ISSPA_TERM_ALL, e->x);
         a.contrib = e->bonded->contrib;
         a.gptr = e->bonded->gptr;
         a.gidx = e->bonded->gidx;
     }
-    if (e->isspa) {
-        CHK(isspa_field_pass(e->isspa, e->x, 0u, e->step_d, s));
-        HIPCHK(launch_force(force_args(e->isspa, x4), e->isspa->npblk, false, s));
-        a.isspa = reduce_args(e->isspa, x4, nullptr, nullptr, nullptr, nullptr);
+    if (e->us) {
+        p.nres = e->us->nres;
+        p.boff = e->us->boff;
+        p.us_atoms = e->us->atoms;
+        p.us_mass = e->us->mass;
+        p.kx = e->us->kx;
+        p.Ls = e->us->L;
+        p.fus = e->fus;
+        p.com = e->com_d;
+        a.fus = e->fus;
+    }
+    HIPCHK(launch_step_prologue(p, s));
+    if (m) {
+        HIPCHK(launch_field(field_args(m, x4), m->nfblk, s));
+        HIPCHK(launch_epilogue(epilogue_args(m, false), s));
+        HIPCHK(launch_force(force_args(m, x4), m->npblk, false, s));
+        a.isspa = reduce_args(m, x4, nullptr, nullptr, nullptr, nullptr);
     }
     if (fork)
         HIPCHK(hipStreamWaitEvent(s, e->ev_join, 0));
-    else if (e->nb)  // after the IS-SPA points launch, whose flags it reuses
-        CHK(nonbond_partials(e->nb, e->x, false, s, e->isspa ? e->isspa->pbc_flag : nullptr));
+    else if (e->nb)  // after the prologue, whose IS-SPA flags it reuses
+        CHK(nonbond_partials(e->nb, e->x, false, s, m ? m->pbc_flag : nullptr));
     if (e->nb) {
         a.nb_wins = e->nb->wins;
         a.nb_atom_win = e->nb->atom_win;
@@ -1115,7 +1153,6 @@ static int engine_forces(md_engine *e, hipStream_t s)
         a.nb_B = e->nb->B;
     }
     HIPCHK(launch_assemble_forces(a, s));
-    if (e->us) CHK(us_force(e->us, e->x, e->f, e->com_d, s));
     return ISSPA_OK;
 }
 

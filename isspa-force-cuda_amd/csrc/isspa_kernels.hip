@@ -26,6 +26,7 @@
 #include "diff_device.hpp"
 #include "kernels.hpp"
 #include "pbc_device.hpp"
+#include "points_device.hpp"
 #include "reduce_device.hpp"
 
 namespace isspa {
@@ -54,63 +55,23 @@ __device__ __forceinline__ void stage_lds(float *dst, const float *__restrict__ 
     for (int t = threadIdx.x; t < n; t += kBlock) dst[t] = src[t];
 }
 
-// x + R * (v 2^-24) with two roundings, never fused into one FMA, so the
-// oracle forms the same bits (v 2^-24 itself is exact)
-__device__ __forceinline__ float point_coord(float x, float R, int v)
-{
-#pragma clang fp contract(off)
-    const float u = (float)v * 5.9604644775390625e-08f;
-    const float du = R * u;
-    return x + du;
-}
-
 // ---------------------------------------------------------------------------
-// A.1 MC points (src/isspa_force_cuda.cu:73-107), Philox instead of curand.
-// Point m of window w (owner m / nMC) is drawn with counter
-// {m, draw, step, sys_w}.  .w of each point carries the owner's rmax^2.
-// The first nflag blocks (one per window, none when nflag = 0) compute
-// instead the per-window minimum-image flags that the field and force passes
-// read, which saves a launch.
+// A.1 MC points: one lane per point (mc_point).  The first nflag blocks (one
+// per window, none when nflag = 0) compute instead the per-window
+// minimum-image flags that the field and force passes read, which saves a
+// launch.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_mc_points(
-    const WinDesc *__restrict__ wins, const int *__restrict__ atom_win,
-    const float4 *__restrict__ xyzq, const int *__restrict__ itype, const float *__restrict__ rmax,
-    const float *__restrict__ rmax2, int ntot, int nmc, uint64_t seed, uint32_t step_arg,
-    const uint32_t *__restrict__ step_d, float4 *__restrict__ pts, int nflag, float rmax_all,
-    int *flag)
+__global__ __launch_bounds__(kBlock) void k_mc_points(PointsArgs a, int nflag, float rmax_all,
+                                                     int *flag)
 {
     if ((int)blockIdx.x < nflag) {
-        const WinDesc W = wins[blockIdx.x];
-        pbc_flag_block(W, xyzq, rmax_all, flag + blockIdx.x);
+        const WinDesc W = a.wins[blockIdx.x];
+        pbc_flag_block(W, a.xyzq, rmax_all, flag + blockIdx.x);
         return;
     }
     const long long gp = (long long)(blockIdx.x - nflag) * kBlock + threadIdx.x;
-    if (gp >= (long long)ntot * nmc) return;
-    const uint32_t step = step_d ? *step_d : step_arg;
-    const int k = (int)(gp / nmc);
-    const WinDesc W = wins[atom_win[k]];
-    const uint32_t m = (uint32_t)(gp - (long long)W.a0 * nmc);
-    const int t = itype[k];
-    int v0 = 0, v1 = 0, v2 = 0;
-    for (int a = 0; a < kMcMaxAttempts; ++a) {
-        const uint4 o = philox_draw(seed, m, kStreamPoints, (uint32_t)a, step, W.sys);
-        const int c0 = 2 * (int)(o.x >> 8) - (1 << 24);
-        const int c1 = 2 * (int)(o.y >> 8) - (1 << 24);
-        const int c2 = 2 * (int)(o.z >> 8) - (1 << 24);
-        const uint64_t r2 = (uint64_t)((int64_t)c0 * c0 + (int64_t)c1 * c1 + (int64_t)c2 * c2);
-        if (r2 <= kMcAcceptMax) {
-            v0 = c0; v1 = c1; v2 = c2;
-            break;
-        }
-    }
-    const float R = rmax[t];
-    const float4 x = xyzq[k];
-    float4 p;
-    p.x = point_coord(x.x, R, v0);
-    p.y = point_coord(x.y, R, v1);
-    p.z = point_coord(x.z, R, v2);
-    p.w = rmax2[t];
-    pts[gp] = p;
+    if (gp >= (long long)a.ntot * a.nmc) return;
+    mc_point(a, gp);
 }
 
 // ---------------------------------------------------------------------------
@@ -608,7 +569,8 @@ __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
     }
 }
 
-// The MD engine's per-step force: f = ((bonded + IS-SPA) + all-pairs) per atom,
+// The MD engine's per-step force: f = ((bonded + IS-SPA) + all-pairs) per atom
+// (+ the restraint force of restrained atoms),
 // each term summed by the same helper and in the same order as its own
 // reduction kernel, and added in the order the separate calls add them into a
 // zeroed f, so the result equals zero + gather + reduce + chunk reduction bit
@@ -659,6 +621,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_forces(AssembleArgs a)
         f.y += (float)nbs[1][al];
         f.z += (float)nbs[2][al];
     }
+    if (a.fus) {
+        const float4 u = a.fus[i];
+        if (u.w != 0.0f) {
+            f.x += u.x;
+            f.y += u.y;
+            f.z += u.z;
+        }
+    }
     a.f[i] = f;
 }
 
@@ -675,9 +645,9 @@ hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const floa
     const long long P = (long long)ntot * nmc;
     const int nflag = flag ? nwin : 0;
     if (P == 0 && nflag == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P) + nflag), dim3(kBlock), 0, s, wins,
-                       atom_win, xyzq, itype, rmax, rmax2, ntot, nmc, seed, step, step_d, pts,
-                       nflag, rmax_all, flag);
+    const PointsArgs a{wins, atom_win, xyzq, itype, rmax, rmax2, ntot, nmc, seed, step, step_d, pts};
+    hipLaunchKernelGGL(k_mc_points, dim3(blocks_for(P) + nflag), dim3(kBlock), 0, s, a, nflag,
+                       rmax_all, flag);
     return hipGetLastError();
 }
 

@@ -124,6 +124,20 @@ struct ReduceArgs {
     double *f_lj, *f_pol, *f_cdd;  // component outputs (split mode)
 };
 
+// MC points: one per (atom, draw) of every window
+struct PointsArgs {
+    const WinDesc *wins;
+    const int *atom_win;
+    const float4 *xyzq;
+    const int *itype;
+    const float *rmax, *rmax2;
+    int ntot, nmc;
+    uint64_t seed;
+    uint32_t step;            // used when step_d is null
+    const uint32_t *step_d;
+    float4 *pts;
+};
+
 hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const float4 *xyzq,
                             const int *itype, const float *rmax, const float *rmax2, int ntot,
                             int nmc, uint64_t seed, uint32_t step, const uint32_t *step_d,
@@ -147,6 +161,7 @@ struct AssembleArgs {
     const int *nb_atom_win;
     const double *nb_part;
     long long nb_B;
+    const float4 *fus;      // restraint bias forces (w = 1 marks a restrained atom), added last
 };
 hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s);
 
@@ -194,6 +209,27 @@ struct BondedArgs {
 hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s);
 hipError_t launch_bonded_gather(const float4 *contrib, const int *ptr, const int *idx, int n,
                                 float4 *f, hipStream_t s);
+// The MD engine's step prologue: every launch of a step that depends only on
+// the coordinates, in one grid.  Each part is skipped when empty (nflag 0,
+// bonded.contrib null, nres 0, pts.ntot 0).
+struct PrologueArgs {
+    int nflag;              // minimum-image flags of the IS-SPA windows
+    float rmax_all;
+    int *flag;
+    BondedArgs bonded;      // bonded terms into their contribution slots
+    int bonded_blocks;      // set by the launcher
+    int nres;               // restraints: bias forces stored into fus (w = 1)
+    const int *boff;
+    const int2 *us_atoms;
+    const float *us_mass;
+    const float2 *kx;
+    const float *Ls;
+    float4 *fus;
+    double *com;
+    PointsArgs pts;         // MC points (pts.wins also serves the flags)
+};
+hipError_t launch_step_prologue(PrologueArgs a, hipStream_t s);
+
 hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, double *out5,
                                 hipStream_t s);
 

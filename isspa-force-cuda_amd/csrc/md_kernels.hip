@@ -10,6 +10,7 @@
 #include "kernels.hpp"
 #include "pbc_device.hpp"
 #include "reduce_device.hpp"
+#include "points_device.hpp"
 
 namespace isspa {
 
@@ -313,9 +314,8 @@ __device__ void dih_term(const BondedArgs &a, int q)
     a.energy[eb + 2] = e14v;
 }
 
-__global__ __launch_bounds__(kBlk) void k_bonded_terms(BondedArgs a)
+__device__ __forceinline__ void bonded_one(const BondedArgs &a, int q)
 {
-    const int q = blockIdx.x * kBlk + threadIdx.x;
     if (q < a.nb) {
         if (a.terms & 1) {
             bond_term(a, q);
@@ -342,6 +342,11 @@ __global__ __launch_bounds__(kBlk) void k_bonded_terms(BondedArgs a)
             for (int s = 0; s < 3; ++s) a.energy[(size_t)a.nb + a.na + 3 * (size_t)k + s] = 0.0;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlk) void k_bonded_terms(BondedArgs a)
+{
+    bonded_one(a, blockIdx.x * kBlk + threadIdx.x);
 }
 
 hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s)
@@ -516,16 +521,17 @@ hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDe
 // reference synchronises only within a block, quirk Q12), then each bias
 // atom's force.  Atoms in a restraint are unique, so the updates do not race.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlk) void k_us(const int *__restrict__ boff,
-                                             const int2 *__restrict__ atoms,
-                                             const float *__restrict__ mass,
-                                             const float2 *__restrict__ kx,
-                                             const float *__restrict__ Ls,
-                                             const float4 *__restrict__ xyzq, float4 *f,
-                                             double *com)
+// restraint r by one block; the bias force of each restrained atom is added
+// into f, or with `store` written to f (w = 1 marking the atom)
+__device__ __forceinline__ void us_block(int r, const int *__restrict__ boff,
+                                         const int2 *__restrict__ atoms,
+                                         const float *__restrict__ mass,
+                                         const float2 *__restrict__ kx,
+                                         const float *__restrict__ Ls,
+                                         const float4 *__restrict__ xyzq, float4 *f, double *com,
+                                         bool store)
 {
     __shared__ double red[6][kBlk];
-    const int r = blockIdx.x;
     const int b0 = boff[r], b1 = boff[r + 1];
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int b = b0 + threadIdx.x; b < b1; b += kBlk) {
@@ -558,13 +564,30 @@ __global__ __launch_bounds__(kBlk) void k_us(const int *__restrict__ boff,
         const int2 at = atoms[b];
         const double kk = at.y == 0 ? -(double)p.x : (double)p.x;
         const double pre = kk * (d - (double)p.y) * (double)mass[b] / d;
-        float4 v = f[at.x];
-        v.x += (float)(pre * dv[0]);
-        v.y += (float)(pre * dv[1]);
-        v.z += (float)(pre * dv[2]);
-        f[at.x] = v;
+        const float fx = (float)(pre * dv[0]), fy = (float)(pre * dv[1]),
+                    fz = (float)(pre * dv[2]);
+        if (store) {
+            f[at.x] = make_float4(fx, fy, fz, 1.0f);
+        } else {
+            float4 v = f[at.x];
+            v.x += fx;
+            v.y += fy;
+            v.z += fz;
+            f[at.x] = v;
+        }
     }
     if (com && threadIdx.x < 6) com[6 * r + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(kBlk) void k_us(const int *__restrict__ boff,
+                                             const int2 *__restrict__ atoms,
+                                             const float *__restrict__ mass,
+                                             const float2 *__restrict__ kx,
+                                             const float *__restrict__ Ls,
+                                             const float4 *__restrict__ xyzq, float4 *f,
+                                             double *com)
+{
+    us_block(blockIdx.x, boff, atoms, mass, kx, Ls, xyzq, f, com, false);
 }
 
 hipError_t launch_us(int nres, const int *boff, const int2 *atoms, const float *mass,
@@ -573,6 +596,46 @@ hipError_t launch_us(int nres, const int *boff, const int2 *atoms, const float *
 {
     if (nres == 0) return hipSuccess;
     hipLaunchKernelGGL(k_us, dim3(nres), dim3(kBlk), 0, s, boff, atoms, mass, kx, L, xyzq, f, com);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// MD engine step prologue: the launches of a step that depend only on the
+// coordinates, as block ranges of one grid -- minimum-image flags (one block
+// per window), bonded terms (256 terms a block), restraints (one block each,
+// bias forces stored for the force assembly) and the MC points.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlk) void k_step_prologue(PrologueArgs a)
+{
+    int b = blockIdx.x;
+    if (b < a.nflag) {
+        const WinDesc W = a.pts.wins[b];
+        pbc_flag_block(W, a.pts.xyzq, a.rmax_all, a.flag + b);
+        return;
+    }
+    b -= a.nflag;
+    if (b < a.bonded_blocks) {
+        bonded_one(a.bonded, b * kBlk + (int)threadIdx.x);
+        return;
+    }
+    b -= a.bonded_blocks;
+    if (b < a.nres) {
+        us_block(b, a.boff, a.us_atoms, a.us_mass, a.kx, a.Ls, a.pts.xyzq, a.fus, a.com, true);
+        return;
+    }
+    b -= a.nres;
+    const long long gp = (long long)b * kBlk + threadIdx.x;
+    if (gp < (long long)a.pts.ntot * a.pts.nmc) mc_point(a.pts, gp);
+}
+
+hipError_t launch_step_prologue(PrologueArgs a, hipStream_t s)
+{
+    const long nterms = (long)a.bonded.nb + a.bonded.na + a.bonded.nd;
+    a.bonded_blocks = a.bonded.contrib ? nblocks(nterms) : 0;
+    const long long P = (long long)a.pts.ntot * a.pts.nmc;
+    const long long grid = (long long)a.nflag + a.bonded_blocks + a.nres + (P + kBlk - 1) / kBlk;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_step_prologue, dim3((unsigned)grid), dim3(kBlk), 0, s, a);
     return hipGetLastError();
 }
 
