@@ -1094,7 +1094,7 @@ extern "C" int md_engine_set_step(md_engine *e, uint32_t step, void *stream)
 // follow, and one assembly launch forms f = bonded + IS-SPA + all-pairs +
 // restraint -- bit for bit what zeroing f and calling bonded_force,
 // isspa_model_force, nonbond_force and us_force in that order give.
-static int engine_forces(md_engine *e, hipStream_t s)
+static int engine_forces(md_engine *e, hipStream_t s, bool integrate)
 {
     const float4 *x4 = reinterpret_cast<const float4 *>(e->x);
     const bool fork = e->side != nullptr;
@@ -1152,24 +1152,29 @@ static int engine_forces(md_engine *e, hipStream_t s)
         a.nb_part = e->nb->part;
         a.nb_B = e->nb->B;
     }
+    if (integrate) {
+        // the assembly launch also integrates each atom and advances the step
+        a.lf.wins = e->wins;
+        a.lf.atom_win = e->atom_win;
+        a.lf.xyzq = reinterpret_cast<float4 *>(e->x);
+        a.lf.vel = reinterpret_cast<float4 *>(e->v);
+        a.lf.kT = e->kT;
+        a.lf.dt = e->dt;
+        a.lf.thr = andersen_threshold(e->pnu);
+        a.lf.seed = e->seed;
+        a.lf.step_d = e->step_d;
+        a.lf.advance = e->step_d;
+    }
     HIPCHK(launch_assemble_forces(a, s));
     return ISSPA_OK;
 }
 
-static int engine_step(md_engine *e, hipStream_t s)
-{
-    CHK(engine_forces(e, s));
-    HIPCHK(launch_leapfrog(e->wins, e->atom_win, WinDesc{}, e->n, reinterpret_cast<float4 *>(e->x),
-                           reinterpret_cast<float4 *>(e->v), reinterpret_cast<const float4 *>(e->f),
-                           e->kT, e->dt, andersen_threshold(e->pnu), e->seed, 0u, e->step_d, s,
-                           e->step_d));
-    return ISSPA_OK;
-}
+static int engine_step(md_engine *e, hipStream_t s) { return engine_forces(e, s, true); }
 
 extern "C" int md_engine_forces(md_engine *e, void *stream)
 {
     if (!e) return fail(ISSPA_ERR_ARG, "null engine");
-    return engine_forces(e, S(stream));
+    return engine_forces(e, S(stream), false);
 }
 
 extern "C" int md_engine_step(md_engine *e, void *stream)

@@ -569,69 +569,6 @@ __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
     }
 }
 
-// The MD engine's per-step force: f = ((bonded + IS-SPA) + all-pairs) per atom
-// (+ the restraint force of restrained atoms),
-// each term summed by the same helper and in the same order as its own
-// reduction kernel, and added in the order the separate calls add them into a
-// zeroed f, so the result equals zero + gather + reduce + chunk reduction bit
-// for bit in one launch.  The last slice also sums the all-pairs partials.
-__global__ __launch_bounds__(kBlock) void k_assemble_forces(AssembleArgs a)
-{
-    constexpr int nc = 9;  // bonded xyz, IS-SPA 6
-    __shared__ double red[kSumSlices - 1][nc][kSumAtoms];
-    __shared__ double nbs[3][kSumAtoms];
-    const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
-    const int i = blockIdx.x * kSumAtoms + al;
-    const bool valid = i < a.n;
-    double s[nc];
-    for (int k = 0; k < nc; ++k) s[k] = 0.0;
-    if (valid) {
-        if (a.contrib) bonded_slice_sum(a.contrib, a.gptr, a.gidx, i, sl, s[0], s[1], s[2]);
-        if (a.isspa.part) {
-            double t[6] = {0, 0, 0, 0, 0, 0};
-            isspa_slice_sum<6>(a.isspa, i, sl, t);
-            for (int k = 0; k < 6; ++k) s[3 + k] = t[k];
-        }
-        if (a.nb_part && sl == kSumSlices - 1) {
-            double t[3] = {0, 0, 0};
-            nonbond_chunk_sum<3>(a.nb_wins, a.nb_atom_win, a.nb_part, a.nb_B, i, t);
-            for (int k = 0; k < 3; ++k) nbs[k][al] = t[k];
-        }
-    }
-    if (sl > 0)
-        for (int k = 0; k < nc; ++k) red[sl - 1][k][al] = s[k];
-    __syncthreads();
-    if (sl != 0 || !valid) return;
-    for (int t = 0; t < kSumSlices - 1; ++t)
-        for (int k = 0; k < nc; ++k) s[k] += red[t][k][al];
-    float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (a.contrib) {
-        f.x += (float)s[0];
-        f.y += (float)s[1];
-        f.z += (float)s[2];
-    }
-    if (a.isspa.part) {
-        const float3 fi = isspa_atom_force(a.isspa, i, s + 3);
-        f.x += fi.x;
-        f.y += fi.y;
-        f.z += fi.z;
-    }
-    if (a.nb_part) {
-        f.x += (float)nbs[0][al];
-        f.y += (float)nbs[1][al];
-        f.z += (float)nbs[2][al];
-    }
-    if (a.fus) {
-        const float4 u = a.fus[i];
-        if (u.w != 0.0f) {
-            f.x += u.x;
-            f.y += u.y;
-            f.z += u.z;
-        }
-    }
-    a.f[i] = f;
-}
-
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -703,12 +640,5 @@ hipError_t launch_force_reduce(const ReduceArgs &a, bool split, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s)
-{
-    if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_assemble_forces, dim3((a.n + kSumAtoms - 1) / kSumAtoms), dim3(kBlock), 0,
-                       s, a);
-    return hipGetLastError();
-}
 
 }  // namespace isspa

@@ -162,6 +162,16 @@ struct AssembleArgs {
     const double *nb_part;
     long long nb_B;
     const float4 *fus;      // restraint bias forces (w = 1 marks a restrained atom), added last
+    struct {                // leap-frog + Andersen of the assembled atoms (xyzq null: none)
+        const WinDesc *wins;    // integrator windows, both required with xyzq
+        const int *atom_win;
+        float4 *xyzq, *vel;
+        float kT, dt;
+        uint32_t thr;
+        uint64_t seed;
+        const uint32_t *step_d;
+        uint32_t *advance;  // {step, blocks done}: the last block moves the step on
+    } lf;
 };
 hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s);
 

@@ -432,18 +432,32 @@ hipError_t launch_bonded_energy(const double *eterm, int nb, int na, int nd, dou
 // velocity is redrawn from Maxwell-Boltzmann, then half-kicked (quirk Q21).
 // Atom i of window w draws with counter {i - a0_w, draw, step, sys_w}.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void leapfrog_atom(const WinDesc *__restrict__ wins,
-                                              const int *__restrict__ atom_win,
-                                              const WinDesc &one, int i, float4 *xyzq,
-                                              float4 *vel, const float4 *__restrict__ f,
-                                              float kT, float dt, uint32_t thr, uint64_t seed,
-                                              uint32_t step)
+// advance = {step, blocks done}, called by every thread of every block after
+// its last read of the step: the last block to get here moves the step on and
+// clears the count.  Every block read the step before its count, so no block
+// of this launch sees the new value.
+__device__ __forceinline__ void advance_step(uint32_t *advance, uint32_t step)
 {
-    const WinDesc W = atom_win ? wins[atom_win[i]] : one;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(advance + 1, 1u);
+        if (done == gridDim.x - 1) {
+            advance[0] = step + 1u;
+            atomicExch(advance + 1, 0u);
+            __threadfence();
+        }
+    }
+}
+
+// atom i of window W
+__device__ __forceinline__ void leapfrog_atom(const WinDesc W, int i, float4 *xyzq, float4 *vel,
+                                              const float4 fi, float kT, float dt, uint32_t thr,
+                                              uint64_t seed, uint32_t step)
+{
     const uint32_t il = (uint32_t)(i - W.a0);
     const uint4 o = philox_draw(seed, il, kStreamAndersen, 0u, step, W.sys);
     float4 v = vel[i];
-    const float4 fi = f[i];
     float4 x = xyzq[i];
     const float m = v.w;
     if (o.x < thr) {
@@ -487,21 +501,10 @@ __global__ __launch_bounds__(kBlk) void k_leapfrog(const WinDesc *__restrict__ w
 {
     const int i = blockIdx.x * kBlk + threadIdx.x;
     const uint32_t step = step_d ? *step_d : step_arg;
-    if (i < ntot) leapfrog_atom(wins, atom_win, one, i, xyzq, vel, f, kT, dt, thr, seed, step);
-    if (!advance) return;
-    // advance = {step, blocks done}: the last block to finish moves the step
-    // on and clears the count.  Every block read the step before its count,
-    // so no block of this launch sees the new value.
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(advance + 1, 1u);
-        if (done == gridDim.x - 1) {
-            advance[0] = step + 1u;
-            atomicExch(advance + 1, 0u);
-            __threadfence();
-        }
-    }
+    if (i < ntot)
+        leapfrog_atom(atom_win ? wins[atom_win[i]] : one, i, xyzq, vel, f[i], kT, dt, thr, seed,
+                      step);
+    if (advance) advance_step(advance, step);
 }
 
 hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDesc &one,
@@ -512,6 +515,95 @@ hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDe
     if (ntot == 0) return advance ? launch_step_advance(advance, s) : hipSuccess;
     hipLaunchKernelGGL(k_leapfrog, dim3(nblocks(ntot)), dim3(kBlk), 0, s, wins, atom_win, one, ntot,
                        xyzq, vel, f, kT, dt, thr, seed, step, step_d, advance);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The MD engine's per-step force: f = ((bonded + IS-SPA) + all-pairs) per atom,
+// plus the restraint force of restrained atoms.  Each term is summed by the
+// same helper and in the same order as its own reduction kernel and added in
+// the order the separate calls add them into a zeroed f, so the result equals
+// zero + gather + IS-SPA reduction + chunk reduction + restraint bit for bit,
+// in one launch.  The last slice also sums the all-pairs partials.  With
+// lf.xyzq the same thread then integrates its atom (leap-frog + Andersen, as
+// k_leapfrog) and the last block advances the step, so a whole step ends with
+// this launch.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlk) void k_assemble_forces(AssembleArgs a)
+{
+    constexpr int nc = 9;  // bonded xyz, IS-SPA 6
+    __shared__ double red[kSumSlices - 1][nc][kSumAtoms];
+    __shared__ double nbs[3][kSumAtoms];
+    const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
+    const int i = blockIdx.x * kSumAtoms + al;
+    const bool valid = i < a.n;
+    const uint32_t step = a.lf.step_d ? *a.lf.step_d : 0u;
+    double s[nc];
+    for (int k = 0; k < nc; ++k) s[k] = 0.0;
+    if (valid) {
+        if (a.contrib) {
+            double bx = 0.0, by = 0.0, bz = 0.0;
+            bonded_slice_sum(a.contrib, a.gptr, a.gidx, i, sl, bx, by, bz);
+            s[0] = bx;
+            s[1] = by;
+            s[2] = bz;
+        }
+        if (a.isspa.part) {
+            double t[6] = {0, 0, 0, 0, 0, 0};
+            isspa_slice_sum<6>(a.isspa, i, sl, t);
+            for (int k = 0; k < 6; ++k) s[3 + k] = t[k];
+        }
+        if (a.nb_part && sl == kSumSlices - 1) {
+            double t[3] = {0, 0, 0};
+            nonbond_chunk_sum<3>(a.nb_wins, a.nb_atom_win, a.nb_part, a.nb_B, i, t);
+            for (int k = 0; k < 3; ++k) nbs[k][al] = t[k];
+        }
+    }
+    if (sl > 0)
+        for (int k = 0; k < nc; ++k) red[sl - 1][k][al] = s[k];
+    __syncthreads();
+    if (sl == 0 && valid) {
+        for (int t = 0; t < kSumSlices - 1; ++t)
+            for (int k = 0; k < nc; ++k) s[k] += red[t][k][al];
+        float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (a.contrib) {
+            f.x += (float)s[0];
+            f.y += (float)s[1];
+            f.z += (float)s[2];
+        }
+        if (a.isspa.part) {
+            const double si[6] = {s[3], s[4], s[5], s[6], s[7], s[8]};
+            const float3 fi = isspa_atom_force(a.isspa, i, si);
+            f.x += fi.x;
+            f.y += fi.y;
+            f.z += fi.z;
+        }
+        if (a.nb_part) {
+            f.x += (float)nbs[0][al];
+            f.y += (float)nbs[1][al];
+            f.z += (float)nbs[2][al];
+        }
+        if (a.fus) {
+            const float4 u = a.fus[i];
+            if (u.w != 0.0f) {
+                f.x += u.x;
+                f.y += u.y;
+                f.z += u.z;
+            }
+        }
+        a.f[i] = f;
+        if (a.lf.xyzq)
+            leapfrog_atom(a.lf.wins[a.lf.atom_win[i]], i, a.lf.xyzq, a.lf.vel, f, a.lf.kT, a.lf.dt,
+                          a.lf.thr, a.lf.seed, step);
+    }
+    if (a.lf.advance) advance_step(a.lf.advance, step);
+}
+
+hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s)
+{
+    if (a.n == 0) return a.lf.advance ? launch_step_advance(a.lf.advance, s) : hipSuccess;
+    hipLaunchKernelGGL(k_assemble_forces, dim3((a.n + kSumAtoms - 1) / kSumAtoms), dim3(kBlk), 0,
+                       s, a);
     return hipGetLastError();
 }
 
