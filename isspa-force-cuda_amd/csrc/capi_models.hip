@@ -632,6 +632,15 @@ extern "C" int nonbond_model_destroy(nonbond_model *m)
     return ISSPA_OK;
 }
 
+static NonbondArgs nonbond_args(nonbond_model *m, const float4 *x, const int *pbc_flag)
+{
+    NonbondArgs a{};
+    a.wins = m->wins; a.blocks = m->blk; a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T;
+    a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
+    a.pbc_flag = pbc_flag; a.part = m->part; a.B = m->B;
+    return a;
+}
+
 int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStream_t s,
                      const int *pbc_flag)
 {
@@ -643,11 +652,7 @@ int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStre
         HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
         pbc_flag = m->pbc_flag;
     }
-    NonbondArgs a{};
-    a.wins = m->wins; a.blocks = m->blk; a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T;
-    a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
-    a.pbc_flag = pbc_flag; a.part = m->part; a.B = m->B;
-    HIPCHK(launch_nonbond(a, m->nblk, split, s));
+    HIPCHK(launch_nonbond(nonbond_args(m, x, pbc_flag), m->nblk, split, s));
     return ISSPA_OK;
 }
 
@@ -1136,16 +1141,24 @@ static int engine_forces(md_engine *e, hipStream_t s, bool integrate)
         a.fus = e->fus;
     }
     HIPCHK(launch_step_prologue(p, s));
+    // without the side stream the all-pairs blocks ride on the field launch,
+    // reusing the prologue's IS-SPA minimum-image flags
+    const bool nb_in_field = m && e->nb && !fork;
     if (m) {
-        HIPCHK(launch_field(field_args(m, x4), m->nfblk, s));
+        FieldArgs fa = field_args(m, x4);
+        if (nb_in_field) {
+            fa.nb = nonbond_args(e->nb, x4, m->pbc_flag);
+            fa.nb_blocks = e->nb->nblk;
+        }
+        HIPCHK(launch_field(fa, m->nfblk, s));
         HIPCHK(launch_epilogue(epilogue_args(m, false), s));
         HIPCHK(launch_force(force_args(m, x4), m->npblk, false, s));
         a.isspa = reduce_args(m, x4, nullptr, nullptr, nullptr, nullptr);
     }
     if (fork)
         HIPCHK(hipStreamWaitEvent(s, e->ev_join, 0));
-    else if (e->nb)  // after the prologue, whose IS-SPA flags it reuses
-        CHK(nonbond_partials(e->nb, e->x, false, s, m ? m->pbc_flag : nullptr));
+    else if (e->nb && !nb_in_field)
+        CHK(nonbond_partials(e->nb, e->x, false, s));
     if (e->nb) {
         a.nb_wins = e->nb->wins;
         a.nb_atom_win = e->nb->atom_win;

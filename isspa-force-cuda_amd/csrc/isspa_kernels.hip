@@ -22,11 +22,14 @@
 //  * the field and force kernels are templated on AN: table lookups (the
 //    plain path) or the analytic per-type functions of the differentiable
 //    model (diff_device.hpp); only the rare in-sphere branches differ.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "diff_device.hpp"
 #include "kernels.hpp"
 #include "pbc_device.hpp"
 #include "points_device.hpp"
+#include "nonbond_device.hpp"
 #include "reduce_device.hpp"
 
 namespace isspa {
@@ -161,9 +164,17 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
     }
 }
 
-template <bool AN>
+// NB: the launch also carries the all-pairs blocks after the field blocks (a
+// separate instantiation, so the plain field loop keeps its own registers)
+template <bool AN, bool NB>
 __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
 {
+    extern __shared__ float lds_tab[];
+    if (NB && (int)blockIdx.x >= a.nfield) {
+        // an all-pairs block riding on this launch (its LJ table in the LDS)
+        nonbond_block<false>(a.nb, blockIdx.x - a.nfield, reinterpret_cast<float2 *>(lds_tab));
+        return;
+    }
     const int4 bt = a.blocks[blockIdx.x];
     const WinDesc W = a.wins[bt.x];
     const int P = W.n * a.nmc;
@@ -176,7 +187,6 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     const float4 p = valid ? a.pts[gp] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     const bool pbc = a.pbc_flag[bt.x] != 0;
     // g and E tables for the in-sphere branch: LDS when they fit
-    extern __shared__ float lds_tab[];
     const float *Gtab = a.tab.g, *Etab = a.tab.e;
     const DiffType *dtypes = a.dtypes;
     if (!AN && field_tab_floats(a.tab) <= kTabLdsFloats) {
@@ -588,17 +598,26 @@ hipError_t launch_mc_points(const WinDesc *wins, const int *atom_win, const floa
     return hipGetLastError();
 }
 
-hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s)
+hipError_t launch_field(const FieldArgs &a0, int nblocks, hipStream_t s)
 {
-    if (nblocks == 0) return hipSuccess;
+    FieldArgs a = a0;
+    a.nfield = nblocks;
     if (a.dtypes) {
+        if (nblocks == 0) return hipSuccess;
         const int nt = diff_tab_floats(a.tab);
         const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
-        hipLaunchKernelGGL(k_field<true>, dim3(nblocks), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((k_field<true, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
     } else {
+        if (nblocks + a.nb_blocks == 0) return hipSuccess;
         const int nt = field_tab_floats(a.tab);
-        const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
-        hipLaunchKernelGGL(k_field<false>, dim3(nblocks), dim3(kBlock), lds, s, a);
+        size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
+        if (a.nb_blocks) {
+            lds = std::max(lds, sizeof(float2) * a.nb.T * a.nb.T);
+            hipLaunchKernelGGL((k_field<false, true>), dim3(nblocks + a.nb_blocks), dim3(kBlock),
+                               lds, s, a);
+        } else {
+            hipLaunchKernelGGL((k_field<false, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
+        }
     }
     return hipGetLastError();
 }

@@ -56,6 +56,22 @@ struct TableView {
 constexpr int kRecFloats = 16;
 constexpr int kAuxFloats = 4;
 
+struct NonbondArgs {
+    const WinDesc *wins;
+    const int4 *blocks;    // {w, receiver block, partner chunk, -}
+    const float4 *xyzq;
+    const float4 *aux;     // per atom j: {4 R_j^2, cA_j, cB_j, LJ type (int bits)}
+    const float2 *lj;      // [T*T] {12A, 6B}
+    int T;
+    const uint32_t *excl;  // [N][4] bit window j - i + 64 in [0,128)
+    const int *excl_ptr;   // CSR fallback (null when the windows are complete)
+    const int *excl_idx;
+    float cfar;
+    const int *pbc_flag;
+    double *part;          // [comps][B]
+    long long B;
+};
+
 struct FieldArgs {
     const WinDesc *wins;
     const int4 *blocks;   // {w, point block, split, -}
@@ -72,6 +88,10 @@ struct FieldArgs {
     double *part_s0;      // [3][F]  sum_out q r / d^3
     double *part_ein;     // [3][F]  sum_in (-e0 q / d^3 + Etab / d) r
     long long F;          // total field partial length
+    // all-pairs partials computed by nb_blocks extra blocks of the same launch
+    // (tables only, nb_blocks = 0: none); nfield is set by the launcher
+    NonbondArgs nb;
+    int nb_blocks, nfield;
 };
 
 struct EpilogueArgs {
@@ -176,21 +196,6 @@ struct AssembleArgs {
 hipError_t launch_assemble_forces(const AssembleArgs &a, hipStream_t s);
 
 // ---- nonbond ----
-struct NonbondArgs {
-    const WinDesc *wins;
-    const int4 *blocks;    // {w, receiver block, partner chunk, -}
-    const float4 *xyzq;
-    const float4 *aux;     // per atom j: {4 R_j^2, cA_j, cB_j, LJ type (int bits)}
-    const float2 *lj;      // [T*T] {12A, 6B}
-    int T;
-    const uint32_t *excl;  // [N][4] bit window j - i + 64 in [0,128)
-    const int *excl_ptr;   // CSR fallback (null when the windows are complete)
-    const int *excl_idx;
-    float cfar;
-    const int *pbc_flag;
-    double *part;          // [comps][B]
-    long long B;
-};
 hipError_t launch_nonbond(const NonbondArgs &a, int nblocks, bool split, hipStream_t s);
 hipError_t launch_chunk_reduce3(const WinDesc *wins, const int *atom_win, const double *part,
                                 long long B, int ntot, int comps, float4 *f, double *o0,
