@@ -557,6 +557,8 @@ __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
         for (int k = 0; k < comps; ++k) red[sl - 1][k][al] = s[k];
     __syncthreads();
     if (sl != 0 || !valid) return;
+    // slices in order; LDS reads of a few slices in flight
+#pragma unroll 4
     for (int t = 0; t < kSumSlices - 1; ++t)
         for (int k = 0; k < comps; ++k) s[k] += red[t][k][al];
     if (!SPLIT) {

@@ -38,18 +38,30 @@ hipError_t launch_nonbond(const NonbondArgs &a, int nb, bool split, hipStream_t 
 
 // sum [comps][B] partials over the window's chunks in order; comps == 3:
 // f += sum (or, with f null, o0 += sum in FP64); comps == 9: three FP64 outputs
+// sum over the partner chunks in the fixed slice order of nonbond_slice_sum
+template <int COMPS>
 __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restrict__ wins,
                                                         const int *__restrict__ atom_win,
                                                         const double *__restrict__ part,
-                                                        long long B, int ntot, int comps,
-                                                        float4 *f, double *o0, double *o1,
-                                                        double *o2)
+                                                        long long B, int ntot, float4 *f,
+                                                        double *o0, double *o1, double *o2)
 {
-    const int i = blockIdx.x * kBlk + threadIdx.x;
-    if (i >= ntot) return;
-    if (comps == 3) {
-        double s[3] = {0, 0, 0};
-        nonbond_chunk_sum<3>(wins, atom_win, part, B, i, s);
+    __shared__ double red[kSumSlices - 1][COMPS][kSumAtoms];
+    const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
+    const int i = blockIdx.x * kSumAtoms + al;
+    const bool valid = i < ntot;
+    double s[COMPS];
+    for (int k = 0; k < COMPS; ++k) s[k] = 0.0;
+    if (valid) nonbond_slice_sum<COMPS>(wins, atom_win, part, B, i, sl, s);
+    if (sl > 0)
+        for (int k = 0; k < COMPS; ++k) red[sl - 1][k][al] = s[k];
+    __syncthreads();
+    if (sl != 0 || !valid) return;
+    // slices in order; LDS reads of a few slices in flight
+#pragma unroll 4
+    for (int t = 0; t < kSumSlices - 1; ++t)
+        for (int k = 0; k < COMPS; ++k) s[k] += red[t][k][al];
+    if constexpr (COMPS == 3) {
         if (f) {
             float4 v = f[i];
             v.x += (float)s[0];
@@ -60,8 +72,6 @@ __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restric
             for (int k = 0; k < 3; ++k) o0[3 * i + k] += s[k];
         }
     } else {
-        double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        nonbond_chunk_sum<9>(wins, atom_win, part, B, i, s);
         for (int k = 0; k < 3; ++k) {
             o0[3 * i + k] = s[k];
             o1[3 * i + k] = s[3 + k];
@@ -75,8 +85,13 @@ hipError_t launch_chunk_reduce3(const WinDesc *wins, const int *atom_win, const 
                                 double *o1, double *o2, hipStream_t s)
 {
     if (ntot == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_chunk_reduce3, dim3(nblocks(ntot)), dim3(kBlk), 0, s, wins, atom_win,
-                       part, B, ntot, comps, f, o0, o1, o2);
+    const dim3 grid((ntot + kSumAtoms - 1) / kSumAtoms);
+    if (comps == 3)
+        hipLaunchKernelGGL(k_chunk_reduce3<3>, grid, dim3(kBlk), 0, s, wins, atom_win, part, B,
+                           ntot, f, o0, o1, o2);
+    else
+        hipLaunchKernelGGL(k_chunk_reduce3<9>, grid, dim3(kBlk), 0, s, wins, atom_win, part, B,
+                           ntot, f, o0, o1, o2);
     return hipGetLastError();
 }
 
@@ -275,6 +290,8 @@ __global__ __launch_bounds__(kBlk) void k_bonded_gather(const float4 *__restrict
     }
     __syncthreads();
     if (sl != 0 || !valid) return;
+    // slices in order; LDS reads of a few slices in flight
+#pragma unroll 4
     for (int t = 0; t < kSumSlices - 1; ++t) {
         sx += red[t][0][al];
         sy += red[t][1][al];
@@ -421,16 +438,15 @@ hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDe
 // same helper and in the same order as its own reduction kernel and added in
 // the order the separate calls add them into a zeroed f, so the result equals
 // zero + gather + IS-SPA reduction + chunk reduction + restraint bit for bit,
-// in one launch.  The last slice also sums the all-pairs partials.  With
+// in one launch.  With
 // lf.xyzq the same thread then integrates its atom (leap-frog + Andersen, as
 // k_leapfrog) and the last block advances the step, so a whole step ends with
 // this launch.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlk) void k_assemble_forces(AssembleArgs a)
 {
-    constexpr int nc = 9;  // bonded xyz, IS-SPA 6
+    constexpr int nc = 12;  // bonded xyz, IS-SPA 6, all-pairs xyz
     __shared__ double red[kSumSlices - 1][nc][kSumAtoms];
-    __shared__ double nbs[3][kSumAtoms];
     const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
     const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < a.n;
@@ -450,17 +466,19 @@ __global__ __launch_bounds__(kBlk) void k_assemble_forces(AssembleArgs a)
             isspa_slice_sum<6>(a.isspa, i, sl, t);
             for (int k = 0; k < 6; ++k) s[3 + k] = t[k];
         }
-        if (a.nb_part && sl == kSumSlices - 1) {
+        if (a.nb_part) {
             double t[3] = {0, 0, 0};
-            nonbond_chunk_sum<3>(a.nb_wins, a.nb_atom_win, a.nb_part, a.nb_B, i, t);
-            for (int k = 0; k < 3; ++k) nbs[k][al] = t[k];
+            nonbond_slice_sum<3>(a.nb_wins, a.nb_atom_win, a.nb_part, a.nb_B, i, sl, t);
+            for (int k = 0; k < 3; ++k) s[9 + k] = t[k];
         }
     }
     if (sl > 0)
         for (int k = 0; k < nc; ++k) red[sl - 1][k][al] = s[k];
     __syncthreads();
     if (sl == 0 && valid) {
-        for (int t = 0; t < kSumSlices - 1; ++t)
+        // slices in order; LDS reads of a few slices in flight
+#pragma unroll 4
+    for (int t = 0; t < kSumSlices - 1; ++t)
             for (int k = 0; k < nc; ++k) s[k] += red[t][k][al];
         float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (a.contrib) {
@@ -476,9 +494,9 @@ __global__ __launch_bounds__(kBlk) void k_assemble_forces(AssembleArgs a)
             f.z += fi.z;
         }
         if (a.nb_part) {
-            f.x += (float)nbs[0][al];
-            f.y += (float)nbs[1][al];
-            f.z += (float)nbs[2][al];
+            f.x += (float)s[9];
+            f.y += (float)s[10];
+            f.z += (float)s[11];
         }
         if (a.fus) {
             const float4 u = a.fus[i];

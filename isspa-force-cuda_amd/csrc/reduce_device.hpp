@@ -13,7 +13,7 @@ namespace isspa {
 // each atom's list: slice t takes entries t, t + S, t + 2S, ... in order, and
 // slice 0 then adds slices 1..S-1 in order.  The slices give the kernel S
 // times the waves of one lane per atom, which small systems need to hide the
-// load latency.
+// load latency (a 44-atom system has 275 IS-SPA chunk partials per atom).
 constexpr int kSumAtoms = 32;
 constexpr int kSumSlices = 8;
 
@@ -68,22 +68,22 @@ __device__ inline void bonded_slice_sum(const float4 *__restrict__ contrib,
     }
 }
 
-// all-pairs partials of atom i (comps = 3, or 9 in split mode) over every
-// partner chunk of its window in chunk order, four chunks' loads in flight
+// all-pairs partials of atom i (COMPS = 3, or 9 in split mode), partner
+// chunks of slice sl; four chunks' loads in flight at a time, added in order
 template <int COMPS>
-__device__ inline void nonbond_chunk_sum(const WinDesc *__restrict__ wins,
+__device__ inline void nonbond_slice_sum(const WinDesc *__restrict__ wins,
                                          const int *__restrict__ atom_win,
                                          const double *__restrict__ part, long long B, int i,
-                                         double (&s)[COMPS])
+                                         int sl, double (&s)[COMPS])
 {
     const WinDesc W = wins[atom_win[i]];
     const long long base = W.npo + (i - W.a0);
-    int c = 0;
-    for (; c + 3 < W.nbc; c += 4) {
+    int c = sl;
+    for (; c + 3 * kSumSlices < W.nbc; c += 4 * kSumSlices) {
         double v[4][COMPS];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const long long o = base + (long long)(c + u) * W.n;
+            const long long o = base + (long long)(c + u * kSumSlices) * W.n;
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) v[u][k] = part[k * B + o];
         }
@@ -92,7 +92,7 @@ __device__ inline void nonbond_chunk_sum(const WinDesc *__restrict__ wins,
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
     }
-    for (; c < W.nbc; ++c) {
+    for (; c < W.nbc; c += kSumSlices) {
         const long long o = base + (long long)c * W.n;
 #pragma unroll
         for (int k = 0; k < COMPS; ++k) s[k] += part[k * B + o];
