@@ -159,8 +159,9 @@ int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int reps, float *m
 /* Tuning knobs (0 = automatic): atom splits of the field pass, point chunks of
  * the force pass, applied to every window.  get_tiling reports the largest
  * window's.  The automatic choice aims at no more than `waves` wavefronts
- * per pass (default 32768, four rounds of the 8192 resident-wave slots),
- * shared among windows in proportion to their work. */
+ * per pass, shared among windows in proportion to their work; without a
+ * set_target_waves call (or after one with waves = 0) that is one wave per 5
+ * MC points, between 2048 and 131072. */
 int isspa_model_set_tiling(isspa_model *m, int field_splits, int force_chunks);
 int isspa_model_get_tiling(const isspa_model *m, int *field_splits, int *force_chunks);
 int isspa_model_set_target_waves(isspa_model *m, int waves);

@@ -111,13 +111,18 @@ static int isspa_tile(isspa_model *m)
     const int nmc = m->d.n_mc;
     double work_tot = 0;
     for (int w = 0; w < L.W; ++w) work_tot += (double)L.n[w] * L.n[w] * nmc;
+    const double target =
+        m->target_waves > 0
+            ? (double)m->target_waves
+            : (double)std::min(kMaxAutoWaves,
+                               std::max(kMinAutoWaves, (long)(m->P / kPointsPerWave)));
     m->wins_h.assign(L.W, WinDesc{});
     long long F = 0, C = 0;
     std::vector<std::pair<long long, int4>> fb, pb;
     for (int w = 0; w < L.W; ++w) {
         WinDesc d = base_desc(L, w);
         const long long n = L.n[w], P = n * nmc;
-        const double tgt = std::max(1.0, m->target_waves * ((double)n * P / work_tot));
+        const double tgt = std::max(1.0, target * ((double)n * P / work_tot));
         int s = m->user_splits > 0 ? m->user_splits : auto_pieces(P, n, kMinSplitAtoms, tgt);
         s = (int)std::max(1LL, std::min((long long)s, n));
         d.aps = (int)((n + s - 1) / s);
@@ -288,7 +293,7 @@ extern "C" int isspa_model_set_tiling(isspa_model *m, int field_splits, int forc
 
 extern "C" int isspa_model_set_target_waves(isspa_model *m, int waves)
 {
-    if (!m || waves < 1) return fail(ISSPA_ERR_ARG, "isspa_model_set_target_waves: bad argument");
+    if (!m || waves < 0) return fail(ISSPA_ERR_ARG, "isspa_model_set_target_waves: bad argument");
     m->target_waves = waves;
     return isspa_tile(m);
 }

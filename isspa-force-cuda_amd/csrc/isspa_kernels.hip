@@ -284,8 +284,36 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     const long long m = gp - (long long)W.a0 * a.nmc;
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
     int cnt = 0;
-    for (int s = 0; s < W.splits; ++s) {
-        const long long o = W.fpo + (long long)s * P + m;
+    // two splits' loads in flight at a time, combined in split order
+    int sp = 0;
+    for (; sp + 1 < W.splits; sp += 2) {
+        float pg[2];
+        int pc[2];
+        double ps[2][3], pe[2][3];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const long long o = W.fpo + (long long)(sp + u) * P + m;
+            pg[u] = a.part_g[o];
+            pc[u] = a.part_cnt[o];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ps[u][c] = a.part_s0[c * a.F + o];
+                pe[u][c] = a.part_ein[c * a.F + o];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            g *= (double)pg[u];
+            cnt += pc[u];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                s0[c] += ps[u][c];
+                ei[c] += pe[u][c];
+            }
+        }
+    }
+    for (; sp < W.splits; ++sp) {
+        const long long o = W.fpo + (long long)sp * P + m;
         g *= (double)a.part_g[o];
         cnt += a.part_cnt[o];
         for (int c = 0; c < 3; ++c) {

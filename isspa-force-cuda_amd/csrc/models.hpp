@@ -13,7 +13,15 @@ using namespace isspa;
 // runs in whole rounds of 8192 waves: targets are multiples of a round and
 // the split never overshoots (a grid of 4.05 rounds takes 5).
 static constexpr long kWaveSlots = 8192;
-static constexpr long kDefaultTargetWaves = 4 * kWaveSlots;
+// Without a target set, one wave per kPointsPerWave MC points, within
+// [kMinAutoWaves, kMaxAutoWaves]: measured on MI355X, more pieces than that
+// cost more in partial sums (the epilogue and the chunk reduction read every
+// piece of a point or atom in turn) than they gain in parallelism (1,100 atoms,
+// nMC = 20: 116 us per step at 4,096 waves against 179 us at 32,768), and
+// the largest systems tune to about 80,000-100,000 waves.
+static constexpr long kPointsPerWave = 5;
+static constexpr long kMinAutoWaves = 2048;
+static constexpr long kMaxAutoWaves = 131072;
 
 struct Windows {
     int W = 0, ntot = 0;
@@ -38,7 +46,7 @@ struct isspa_model {
     int *pbc_flag = nullptr;
     // tiling
     int user_splits = 0, user_chunks = 0;
-    long target_waves = kDefaultTargetWaves;
+    long target_waves = 0;  // 0: automatic (kPointsPerWave)
     std::vector<WinDesc> wins_h;
     WinDesc *wins = nullptr;
     int4 *fblk = nullptr, *pblk = nullptr;
