@@ -111,8 +111,8 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
     W = len(windows)
     g = GpuSystem(merged, "cuda:0")
     tr = autotune.tune(g, reps=tune_reps, waves=(2048, 4096, 8192, 16384, 32768, 65536, 131072),
-                       log=log)
-    log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per IS-SPA pass")
+                       log=log, confirm=autotune.engine_forces_ms)
+    log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per {tr.measure}")
     K = max(1, int(per_graph))
     steps = max(K, steps - steps % K)
     run_plain_loop = plain_rounds > 0
@@ -138,7 +138,8 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
         return {"windows": W, "atoms": merged.n_atoms, "steps": steps, "steps_per_graph": K,
                 "ms_per_step": round(ms_b, 5),
                 "window_ns_per_day": round(W * 0.1728 * steps / t_b, 2),
-                "autotune": {"kind": tr.kind, "value": list(tr.value), "pass_ms": round(tr.ms, 5)}}
+                "autotune": {"kind": tr.kind, "value": list(tr.value), "ms": round(tr.ms, 5),
+                             "measure": tr.measure}}
     plain = []
     for w, s in enumerate(windows):
         p = GpuSystem(s, "cuda:0", system_id=w)
@@ -171,8 +172,8 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
         "plain_loop_ms_per_step": round(ms_p, 5),
         "plain_loop_window_ns_per_day": round(nsd_p, 2),
         "speedup_vs_plain_loop": round(nsd_b / nsd_p, 3),
-        "autotune": {"kind": tr.kind, "value": list(tr.value), "pass_ms": round(tr.ms, 5),
-                     "candidates": len(tr.table)},
+        "autotune": {"kind": tr.kind, "value": list(tr.value), "ms": round(tr.ms, 5),
+                     "measure": tr.measure, "candidates": len(tr.table)},
     }
     for p, _ in plain:
         p.close()
