@@ -981,7 +981,9 @@ extern "C" int us_force(us_model *m, const float *xyzq4_d, float *f4_d, double *
 
 // ---------------------------------------------------------------------------
 // MD engine: one step = zero f, bonded, IS-SPA, nonbond, US, leap-frog, step++
-// (order of src/total_force_cuda.cpp:109-153)
+// (order of src/total_force_cuda.cpp:109-153), as five launches: prologue,
+// field (+ all-pairs below kForkMinAtoms), epilogue, force, assembly (which
+// also integrates and advances the step); see engine_forces.
 // ---------------------------------------------------------------------------
 struct md_engine {
     isspa_model *isspa = nullptr;
