@@ -140,6 +140,22 @@ def test_engine_forces_and_step(cfg0):
     g.close()
 
 
+@pytest.mark.parametrize("name", ["adi2", "cfg1"])
+def test_standalone_calls_equal_the_engine_bitwise(name, adi2, cfg1):
+    """Without the side stream: zero + bonded_force + nonbond_force give the
+    engine's forces (prologue, all-pairs partials, assembly) bit for bit."""
+    s = {"adi2": adi2, "cfg1": cfg1}[name]
+    g = _gpu(s, isspa=False, us=False)
+    g.set_step(0)
+    got = g.forces().clone()
+    g.zero_forces()
+    g.bonded_force()
+    g.nonbond_force()
+    torch.cuda.synchronize()
+    assert torch.equal(got, g.f)
+    g.close()
+
+
 @pytest.mark.parametrize("mode", ["eager", "graph"])
 def test_engine_step_counter_advances_once_per_step(cfg1, mode):
     """The engine keeps its step on the device and the leap-frog launch moves
