@@ -25,10 +25,11 @@ Two candidate families:
 The three fastest candidates are timed again with three times the
 repetitions and the fastest of that second timing is kept; the choice is
 deterministic given the timings (ties keep the earlier candidate).  With
-`confirm` (for example engine_forces_ms) that second timing measures what the
-caller will run, such as the MD engine's whole force evaluation, whose
-all-pairs kernel overlaps the IS-SPA passes on a second stream and so favours
-some tilings over others with the same IS-SPA time.  Changing the tiling rebuilds the launch tables, so tune before
+`confirm` (for example engine_forces_ms) that second timing covers the five
+fastest and measures what the caller will run, such as the MD engine's whole
+force evaluation, whose all-pairs kernel overlaps the IS-SPA passes on a
+second stream and so favours some tilings over others with the same IS-SPA
+time.  Changing the tiling rebuilds the launch tables, so tune before
 capturing a graph.
 """
 from __future__ import annotations
@@ -161,7 +162,9 @@ def tune(g, reps: int = 3, waves: Sequence[int] = (8192, 16384, 24576, 32768, 49
     # the leading candidates differ by about the timing noise: time the best
     # three again with three times the repetitions and decide on that
     final = []
-    for k in sorted(whole, key=lambda k: (table[k][2], k))[:3]:
+    # a confirmation that measures more than the IS-SPA pass re-ranks more
+    # candidates: the pass ranking alone is a noisier guide to it
+    for k in sorted(whole, key=lambda k: (table[k][2], k))[:5 if confirm else 3]:
         kind, value, _ = table[k]
         _apply(g, kind, value)
         ms = confirm(g, 3 * reps) if confirm else _measure(g, 3 * reps)

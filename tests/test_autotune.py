@@ -72,8 +72,8 @@ def test_tune_picks_fastest_pass_and_confirm_overrides():
     r = autotune.tune(g, reps=1)
     assert r.measure == "IS-SPA pass"
     assert g.tiling() == (8, 64) and abs(r.ms - 3.2) < 1e-9
-    # the leaders are waves 32768 and tiling (8, 64) (both 3.2 ms) and tiling
-    # (8, 32) (3.232 ms); a confirmation that prefers (8, 32) decides
+    # the five leaders include waves 32768 and tiling (8, 64) (both 3.2 ms)
+    # and tiling (8, 32) (3.232 ms); a confirmation that prefers (8, 32) decides
     calls = []
 
     def confirm(model, reps):
@@ -83,6 +83,6 @@ def test_tune_picks_fastest_pass_and_confirm_overrides():
     g2 = _FakeModel()
     r2 = autotune.tune(g2, reps=1, confirm=confirm)
     assert r2.measure == "engine forces"
-    assert sorted(calls) == [(8, 32), (8, 64), (8, 64)]
+    assert len(calls) == 5 and (8, 32) in calls and calls.count((8, 64)) == 2
     assert r2.kind == "tiling" and r2.value == (8, 32)
     assert g2.tiling() == (8, 32) and r2.ms == 1.0
