@@ -236,3 +236,49 @@ def test_engine_side_stream_path():
     assert torch.equal(eager.x, graph.x) and torch.equal(eager.v, graph.v)
     eager.close()
     graph.close()
+
+
+def test_full_size_invariants():
+    """19,998 atoms, nMC = 20 (the benchmark workload), through properties
+    that hold at any size.
+    * The LJ and Coulomb pair forces and the bonded forces each sum to zero
+      (Newton's third law, up to FP32 rounding).
+    * Translation, bit for bit: with every coordinate in [256, 512), where
+      FP32 values lie on a 2^-15 grid, a shift by a multiple of 2^-14 is exact,
+      the MC points (drawn relative to their owners with the same counters)
+      shift exactly too, and every separation is the same float.  So every
+      force term must come out identical; a wrong image, index or counter
+      would change it."""
+    from isspa_amd import synth
+    s = copy.deepcopy(synth.workload(2))
+    s.xyzq[:, :3] = (s.xyzq[:, :3].astype(np.float64) + 150.0).astype(np.float32)
+    assert s.xyzq[:, :3].min() > 256 + 12 and s.xyzq[:, :3].max() < 512 - 12
+    print(f"N={s.n_atoms} nMC={s.n_mc}", flush=True)
+
+    def terms(system):
+        g = _gpu(system, us=False)
+        nb = {k: v.cpu().numpy() for k, v in g.nonbond_components().items()}
+        g.zero_forces()
+        fb = g.bonded_force()[0][:, :3].double().cpu().numpy()
+        out = {**{k: v.cpu().numpy() for k, v in g.isspa_components(step=2).items()},
+               "nb_lj": nb["lj"], "nb_coul": nb["coul"], "nb_pair": nb["pair"], "bonded": fb}
+        g.close()
+        return out
+
+    base = terms(s)
+    for k in ("nb_lj", "nb_coul"):
+        net = np.abs(base[k].sum(axis=0)).max()
+        scale = np.abs(base[k]).sum(axis=0).max()
+        print(f"  {k}: |sum F| {net:.3e} of sum |F| {scale:.3e}", flush=True)
+        assert net <= 1e-6 * scale, k
+    fb = base["bonded"]
+    assert np.abs(fb.sum(axis=0)).max() <= 1e-5 * np.abs(fb).sum(axis=0).max()
+    t = copy.deepcopy(s)
+    shift = np.array([3, -5, 7], np.float32) * np.float32(2.0 ** -10)
+    t.xyzq[:, :3] = s.xyzq[:, :3] + shift
+    assert np.array_equal(t.xyzq[:, :3] - s.xyzq[:, :3], np.broadcast_to(shift, (s.n_atoms, 3)))
+    moved = terms(t)
+    for k in base:
+        same = np.array_equal(moved[k], base[k])
+        print(f"  translated {k}: identical {same}", flush=True)
+        assert same, k
