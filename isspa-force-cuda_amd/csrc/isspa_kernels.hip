@@ -164,6 +164,38 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
     }
 }
 
+// a point's sums over atoms [j0, j1) of its split
+struct FieldSums {
+    double sx = 0.0, sy = 0.0, sz = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
+    FieldRare st{1.0f, 0, 0.0f, 0.0f, 0.0f};
+};
+
+template <bool AN>
+__device__ __forceinline__ void field_span(const FieldArgs &a, const float *Gtab,
+                                           const float *Etab, const DiffType *dtypes, int j0,
+                                           int j1, const WinDesc &W, bool pbc, float4 p,
+                                           FieldSums &fs)
+{
+    FieldRare &st = fs.st;
+    for (int jt = j0; jt < j1; jt += kFieldTile) {
+        const int je = min(j1, jt + kFieldTile);
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        st.ex = st.ey = st.ez = 0.0f;
+        if (pbc)
+            field_tile<true, AN>(a, Gtab, Etab, dtypes, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax,
+                                 ay, az, st);
+        else
+            field_tile<false, AN>(a, Gtab, Etab, dtypes, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax,
+                                  ay, az, st);
+        fs.sx += (double)ax;
+        fs.sy += (double)ay;
+        fs.sz += (double)az;
+        fs.ex += (double)st.ex;
+        fs.ey += (double)st.ey;
+        fs.ez += (double)st.ez;
+    }
+}
+
 // NB: the launch also carries the all-pairs blocks after the field blocks (a
 // separate instantiation, so the plain field loop keeps its own registers)
 template <bool AN, bool NB>
@@ -186,41 +218,26 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     const long long gp = (long long)W.a0 * a.nmc + m;
     const float4 p = valid ? a.pts[gp] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     const bool pbc = a.pbc_flag[bt.x] != 0;
-    // g and E tables for the in-sphere branch: LDS when they fit
-    const float *Gtab = a.tab.g, *Etab = a.tab.e;
-    const DiffType *dtypes = a.dtypes;
+    // g and E tables (or the analytic records) for the in-sphere branch: LDS
+    // when they fit.  Separate call sites give each table pointer one address
+    // space, so the LDS ones compile to LDS reads.
+    FieldSums fs;
     if (!AN && field_tab_floats(a.tab) <= kTabLdsFloats) {
         const int ng = a.tab.ntypes * a.tab.ld_g;
         stage_lds(lds_tab, a.tab.g, ng);
         stage_lds(lds_tab + ng, a.tab.e, a.tab.ntypes * a.tab.ld_e);
         __syncthreads();
-        Gtab = lds_tab;
-        Etab = lds_tab + ng;
-    }
-    if (AN && diff_tab_floats(a.tab) <= kTabLdsFloats) {
+        field_span<AN>(a, lds_tab, lds_tab + ng, a.dtypes, j0, j1, W, pbc, p, fs);
+    } else if (AN && diff_tab_floats(a.tab) <= kTabLdsFloats) {
         stage_lds(lds_tab, reinterpret_cast<const float *>(a.dtypes), diff_tab_floats(a.tab));
         __syncthreads();
-        dtypes = reinterpret_cast<const DiffType *>(lds_tab);
+        field_span<AN>(a, a.tab.g, a.tab.e, reinterpret_cast<const DiffType *>(lds_tab), j0, j1,
+                       W, pbc, p, fs);
+    } else {
+        field_span<AN>(a, a.tab.g, a.tab.e, a.dtypes, j0, j1, W, pbc, p, fs);
     }
-    double sx = 0.0, sy = 0.0, sz = 0.0, ex = 0.0, ey = 0.0, ez = 0.0;
-    FieldRare st{1.0f, 0, 0.0f, 0.0f, 0.0f};
-    for (int jt = j0; jt < j1; jt += kFieldTile) {
-        const int je = min(j1, jt + kFieldTile);
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        st.ex = st.ey = st.ez = 0.0f;
-        if (pbc)
-            field_tile<true, AN>(a, Gtab, Etab, dtypes, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax,
-                                 ay, az, st);
-        else
-            field_tile<false, AN>(a, Gtab, Etab, dtypes, jt, je, W.L, W.h, p.x, p.y, p.z, p.w, ax,
-                                  ay, az, st);
-        sx += (double)ax;
-        sy += (double)ay;
-        sz += (double)az;
-        ex += (double)st.ex;
-        ey += (double)st.ey;
-        ez += (double)st.ez;
-    }
+    const FieldRare &st = fs.st;
+    const double sx = fs.sx, sy = fs.sy, sz = fs.sz, ex = fs.ex, ey = fs.ey, ez = fs.ez;
     if (!valid) return;
     const long long o = W.fpo + (long long)split * P + m;
     a.part_g[o] = st.g;
@@ -502,6 +519,32 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
     }
 }
 
+// points [m0, m1) of one chunk for one receiver, in tiles of kForceTile
+template <bool SPLIT, bool AN>
+__device__ __forceinline__ void force_chunk(const ForceArgs &a, const float4 *__restrict__ rec,
+                                            const float4 *__restrict__ aux, int m0, int m1,
+                                            const WinDesc &W, bool pbc, float4 xi, float R2,
+                                            const float *Ftab, const DiffFParams &fp,
+                                            ForceAcc &acc)
+{
+    for (int mt = m0; mt < m1; mt += kForceTile) {
+        const int me = min(m1, mt + kForceTile);
+        TileSums ts{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (pbc)
+            force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
+                                        fp, ts, acc);
+        else
+            force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
+                                         fp, ts, acc);
+        acc.ex += (double)ts.sx;
+        acc.ey += (double)ts.sy;
+        acc.ez += (double)ts.sz;
+        acc.lx += (double)ts.lx;
+        acc.ly += (double)ts.ly;
+        acc.lz += (double)ts.lz;
+    }
+}
+
 template <bool SPLIT, bool AN>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 {
@@ -522,35 +565,20 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     // the receiver type's F table for the in-sphere branch: LDS when it fits;
     // the analytic model keeps its four F constants in registers
     extern __shared__ float lds_tab[];
-    const float *Ftab = nullptr;
     DiffFParams fp{0.0f, 0.0f, 0.0f, 0.0f};
     if (AN) fp = diff_f_params(a.dtypes[ti]);
-    if (!AN) {
-        const float *ftab = a.tab.f;
-        if (force_tab_floats(a.tab) <= kTabLdsFloats) {
-            stage_lds(lds_tab, a.tab.f, force_tab_floats(a.tab));
-            __syncthreads();
-            ftab = lds_tab;
-        }
-        Ftab = ftab + ti * a.tab.ld_f;
-    }
     const bool pbc = a.pbc_flag[bt.x] != 0;
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int mt = m0; mt < m1; mt += kForceTile) {
-        const int me = min(m1, mt + kForceTile);
-        TileSums ts{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (pbc)
-            force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
-                                        fp, ts, acc);
-        else
-            force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
-                                         fp, ts, acc);
-        acc.ex += (double)ts.sx;
-        acc.ey += (double)ts.sy;
-        acc.ez += (double)ts.sz;
-        acc.lx += (double)ts.lx;
-        acc.ly += (double)ts.ly;
-        acc.lz += (double)ts.lz;
+    // two call sites, so the table pointer of each has one address space and
+    // the LDS one compiles to LDS reads
+    if (!AN && force_tab_floats(a.tab) <= kTabLdsFloats) {
+        stage_lds(lds_tab, a.tab.f, force_tab_floats(a.tab));
+        __syncthreads();
+        force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f, fp,
+                               acc);
+    } else {
+        force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2,
+                               AN ? nullptr : a.tab.f + ti * a.tab.ld_f, fp, acc);
     }
     if (!valid) return;
     const long long o = W.ppo + (long long)chunk * W.n + il;
