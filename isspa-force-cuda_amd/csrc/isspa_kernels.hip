@@ -571,12 +571,17 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
     // two call sites, so the table pointer of each has one address space and
     // the LDS one compiles to LDS reads
+    // a wave with no receiver of the window (small windows) stops after the
+    // block's one barrier instead of running the point loop for nothing
+    const bool wave_idle = bt.y * kBlock + (int)(threadIdx.x & ~(kWave - 1)) >= W.n;
     if (!AN && force_tab_floats(a.tab) <= kTabLdsFloats) {
         stage_lds(lds_tab, a.tab.f, force_tab_floats(a.tab));
         __syncthreads();
+        if (wave_idle) return;
         force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f, fp,
                                acc);
     } else {
+        if (wave_idle) return;
         force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2,
                                AN ? nullptr : a.tab.f + ti * a.tab.ld_f, fp, acc);
     }
