@@ -23,6 +23,11 @@ step, back to back on one stream).  Both report aggregate window-ns/day
 (sum over windows of ns/day at dt = 2 fs); the launch geometry of the batched
 model is chosen by isspa_amd.autotune.
 
+The reference's own published run ("reference_published_run"): ADI2, 44
+atoms, nMC=100, one distance restraint, the whole MD step on the engine at
+10 steps per graph replay, in ns/day beside the GTX 1080 figure of
+timing_tests/ADI2_US/window_6.5/ADI2.run0.log (also alone: --adi2-only).
+
 CPU baseline: the reference's own src/isspa_force.omp.c (built in place into
 oracle/_ref by oracle/Makefile), cpu_ispa_force on the same 19,998-atom
 geometry with nMC=20 and the workload's per-type parabola and LJ-solvent
@@ -246,7 +251,8 @@ def main():
     ap.add_argument("--adi2-only", action="store_true",
                     help="run only the reference's published ADI2 configuration on the engine "
                          "and print its JSON object")
-    ap.add_argument("--adi2-steps", type=int, default=20000)
+    ap.add_argument("--adi2-steps", type=int, default=20000,
+                    help="steps of the ADI2 configuration (also timed beside the headline)")
     ap.add_argument("--tiling", default="", help="explicit field splits,force chunks (no tuning)")
     ap.add_argument("--target-waves", type=int, default=0, help="automatic tiling at this target")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -384,6 +390,10 @@ def main():
     if not args.no_batch:
         out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,
                                        tune_reps=3, per_graph=args.steps_per_graph)
+        # the one configuration the reference publishes timings for (ns/day on
+        # a GTX 1080), beside the headline: a different metric, so it does not
+        # set vs_baseline
+        out["reference_published_run"] = adi2_bench(args.adi2_steps, 200, 10)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         res = cpu_baseline(s, threads)
