@@ -381,6 +381,8 @@ __global__ __launch_bounds__(kB) void k_vjp_atoms(VjpArgs a)
     const float4 lp = a.arec[4 * (size_t)j + 2];  // Lambda' = -vtot lambda
     const bool pbc = a.pbc_flag[bt.x] != 0;
     const DiffType &Tj = stage_dtypes(a)[tj];
+    // a wave with no atom of the window stops after the block's barrier
+    if (bt.y * kB + (int)(threadIdx.x & ~(kWave - 1)) >= W.n) return;
     double qacc = 0.0, tb[5] = {0, 0, 0, 0, 0};
     for (int mt = m0; mt < m1; mt += kTile) {
         const int me = min(m1, mt + kTile);
