@@ -259,7 +259,7 @@ def main():
                     help="run an untimed copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
               "(the per-GPU shard design is documented, no multi-GPU path runs here)",
               file=sys.stderr)
         sys.exit(2)

@@ -22,7 +22,7 @@ def _gpu(system, **kw):
 def test_nonbond_golden_adi2(adi2):
     """timing_tests/ADI2/10000MC/compare/ADI2.multi.frc.{LJsolu,Csolu,Csolv_pair}.dat:
     FP64 golden per-atom forces.  The reference's own FP32 GPU output matches
-    them to <= 2.8e-5 absolute; this build must do at least as well."""
+    them to <= 2.8e-5 absolute; the HIP path must do at least as well."""
     g = _gpu(adi2, isspa=False, bonded=False)
     comp = g.nonbond_components()
     for k, name in (("lj", "LJsolu"), ("coul", "Csolu"), ("pair", "Csolv_pair")):
