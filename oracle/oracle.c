@@ -1,7 +1,7 @@
 /*
  * oracle.c -- FP64 CPU restatement of the IS-SPA MD hot path.
  *
- * TEST INFRASTRUCTURE ONLY: this file is the checker the HIP kernels are
+ * TEST INFRASTRUCTURE ONLY: this file is the oracle the HIP kernels are
  * compared against.  It is never linked into the product library.  See
  * oracle.h for the list of reference files each function follows.
  *

@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU restatement of the IS-SPA MD hot path (TEST INFRASTRUCTURE ONLY).
  *
- * This library is the parity checker for the HIP path.  Only tests/, the smoke
+ * This library is the parity oracle for the HIP path.  Only tests/, the smoke
  * entry point and bench.py's cpu_baseline leg may load it; the product path
  * never calls it.
  *

@@ -1,7 +1,7 @@
 /*
  * oracle_diff.c -- FP64 CPU restatement of the differentiable IS-SPA +
  * nonbond force model and its adjoint (TEST INFRASTRUCTURE ONLY: the parity
- * checker of the device adjoint; the product path never calls it).
+ * oracle of the device adjoint; the product path never calls it).
  *
  * The model is the plain path (src/isspa_force_cuda.cu:110-327 field and
  * force passes, src/nonbond_force_cuda.cu:31-174 all-pairs term) with the
