@@ -377,15 +377,20 @@ def main():
                                      "ms": round(tune.ms, 4), "measure": tune.measure,
                                      "candidates": len(tune.table)}
     # HBM-side bytes per k_force launch from the committed FETCH_SIZE /
-    # WRITE_SIZE passes (copy-calibrated); they depend on the tiling, which is
-    # reported with them
+    # WRITE_SIZE passes (copy-calibrated).  k_force's traffic depends on the
+    # force chunk count only (it writes 6 FP64 partials per chunk and atom), so
+    # the figure is reported when this run's chunk count is the profiled one
     tr_path = os.path.join(ROOT, "profiles", "k_force_19998atoms_nmc20_traffic.json")
     if args.workload == 2 and os.path.exists(tr_path):
         with open(tr_path) as fh:
             tr = json.load(fh)
-        out["roofline"]["traffic"] = tr["traffic_bytes"]
+        same = int(tr["tiling"][1]) == chunks
+        out["roofline"]["traffic"] = tr["traffic_bytes"] if same else None
         out["roofline"]["traffic_tiling"] = tr["tiling"]
         out["roofline"]["traffic_source"] = tr["summary"]
+        if not same:
+            out["roofline"]["traffic_note"] = (f"profiled at {tr['tiling'][1]} force chunks; "
+                                               f"this run used {chunks}")
     g.close()
     if not args.no_batch:
         out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,
