@@ -57,9 +57,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(system, threads, budget_s=30.0):
-    """The reference OpenMP cpu_ispa_force on the workload geometry.  Returns
-    (seconds per call, kind, trials) or None if neither library is built."""
+def cpu_model():
+    """the host CPU's model name (/proc/cpuinfo), or the platform string"""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def host_threads():
+    """threads the CPU baseline uses: the cores this process may run on, at
+    most 16 (the host share of one GPU on the measurement machines)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(system, threads, trials=3, budget_s=30.0):
+    """The reference OpenMP cpu_ispa_force on the workload geometry: one call
+    is one step's IS-SPA solvent force over N x P point-atom pairs.  Returns
+    (median seconds per call, kind, trials run) or None if neither library is
+    built."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -86,7 +110,7 @@ def cpu_baseline(system, threads, budget_s=30.0):
                           seed=1)
         times.append(time.perf_counter() - t0)
         log(f"cpu baseline trial {len(times)}: {times[-1]:.2f} s ({threads} threads, {kind})")
-        if len(times) >= 3 or time.perf_counter() - t_start + times[-1] > budget_s:
+        if len(times) >= trials or time.perf_counter() - t_start + times[-1] > budget_s:
             break
     times.sort()
     return times[len(times) // 2], kind, len(times)
@@ -355,6 +379,7 @@ def main():
                         "all-pairs nonbond + leap-frog/Andersen, one hipGraph replay per step",
             "n_atoms": N, "n_mc": s.n_mc, "mc_points": P, "dt_fs": 2.0,
             "field_splits": splits, "force_chunks": chunks,
+            "seeds": synth.workload_seeds(args.workload),
         },
         "ns_per_day": round(0.1728 * steps_per_s, 4),
         "step_tflops": round(step_flop * steps_per_s / 1e12, 3),
@@ -400,17 +425,30 @@ def main():
         # set vs_baseline
         out["reference_published_run"] = adi2_bench(args.adi2_steps, 200, 10)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the reference's own OpenMP file times 1..10 threads
+        # (src/isspa_force.omp.c:172-199): here 1 thread (one call) and the
+        # host's core share (median of 3 calls)
+        threads = args.cpu_threads or host_threads()
+        pairs = N * P
         res = cpu_baseline(s, threads)
+        res1 = cpu_baseline(s, 1, trials=1) if threads > 1 else res
         if res is not None:
             t_cpu, kind, trials = res
+            t_one = res1[0]
             out["cpu_baseline"] = {
                 "value": round(1.0 / t_cpu, 5), "unit": "steps/s", "cores": threads,
                 "kind": kind,
-                "sample": f"cpu_ispa_force (src/isspa_force.omp.c) once per trial on the same "
-                          f"{N}-atom geometry, nMC={s.n_mc}, median of {trials} trial(s); "
+                "sample": f"cpu_ispa_force (src/isspa_force.omp.c) on the same {N}-atom geometry, "
+                          f"nMC={s.n_mc}, one whole step per call ({pairs:.3e} point-atom pairs); "
+                          f"median of {trials} call(s) at {threads} threads, one call at 1 thread; "
                           "LJ-only single-pass model",
                 "seconds_per_step": round(t_cpu, 3),
+                "pair_evals_per_s": round(pairs / t_cpu, 1),
+                "single_thread": {"seconds_per_step": round(t_one, 3),
+                                  "pair_evals_per_s": round(pairs / t_one, 1),
+                                  "speedup_at_cores": round(t_one / t_cpu, 2)},
+                "cpu_model": cpu_model(),
+                "nproc": os.cpu_count(),
             }
             out["gpu_over_cpu_step_ratio"] = round(t_cpu / (ms_step * 1e-3), 1)
     print(json.dumps(out), flush=True)

@@ -348,8 +348,8 @@ ForceArgs force_args(isspa_model *m, const float4 *x)
     return fa;
 }
 
-static ReduceArgs reduce_args(isspa_model *m, const float4 *x, float4 *f, double *flj,
-                              double *fpol, double *fcdd)
+ReduceArgs reduce_args(isspa_model *m, const float4 *x, float4 *f, double *flj, double *fpol,
+                       double *fcdd)
 {
     ReduceArgs ra{};
     ra.wins = m->wins; ra.atom_win = m->atom_win; ra.part = m->part_f; ra.C = m->C;
@@ -777,6 +777,7 @@ extern "C" int bonded_model_create(int n, int nb, const int *bond_atoms, int nbt
         gidx.insert(gidx.end(), slots[i].begin(), slots[i].end());
         gptr[i + 1] = (int)gidx.size();
     }
+    if (gidx.empty()) gidx.push_back(0);  // no terms: one unread entry to upload
     std::vector<float2> lj;
     if (nd && T > 0) {
         lj.resize((size_t)T * T);
@@ -812,7 +813,7 @@ extern "C" int bonded_model_create(int n, int nb, const int *bond_atoms, int nbt
     TRY(dalloc(&m->contrib, (size_t)2 * nb + 3 * na + 6 * nd));
     TRY(dalloc(&m->energy, (size_t)nb + na + 3 * nd));
     TRY(upload(&m->gptr, gptr.data(), gptr.size()));
-    TRY(upload(&m->gidx, gidx.data(), std::max<size_t>(1, gidx.size())));
+    TRY(upload(&m->gidx, gidx.data(), gidx.size()));
 #undef TRY
     *out = m;
     return ISSPA_OK;
@@ -1003,6 +1004,10 @@ struct md_engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     unsigned captured_generation = 0;
+    // recorded after every replay: a recapture or destroy waits on it before
+    // freeing the executable graph, which may still be running
+    hipEvent_t ev_replayed = nullptr;
+    bool replay_pending = false;
     // the all-pairs kernel runs on a side stream beside the IS-SPA passes
     // (default priority; fork and join by events, which a capture records as
     // graph edges); its chunk reduction runs after the join
@@ -1012,12 +1017,26 @@ struct md_engine {
 
 constexpr int kForkMinAtoms = 4096;
 
-static void engine_free(md_engine *e)
+// Destroys the executable graph and its graph once the last replay has ended
+// (a replay still in flight reads the executable's kernel arguments).
+static int engine_drop_graph(md_engine *e)
 {
+    hipError_t he = hipSuccess;
+    if (e->replay_pending && e->ev_replayed) he = hipEventSynchronize(e->ev_replayed);
+    e->replay_pending = false;
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->exec = nullptr;
     e->graph = nullptr;
+    if (he != hipSuccess) return fail(ISSPA_ERR_HIP, "waiting for the last replay: %s", hipGetErrorString(he));
+    return ISSPA_OK;
+}
+
+static void engine_free(md_engine *e)
+{
+    (void)engine_drop_graph(e);
+    if (e->ev_replayed) (void)hipEventDestroy(e->ev_replayed);
+    e->ev_replayed = nullptr;
     if (e->side) (void)hipStreamDestroy(e->side);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
@@ -1066,7 +1085,8 @@ extern "C" int md_engine_create(isspa_model *isspa, nonbond_model *nb, bonded_mo
     }
     if (rc == ISSPA_OK) {
         hipError_t he = hipMemset(e->step_d, 0, 2 * sizeof(uint32_t));
-        if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "hipMemset: %s", hipGetErrorString(he));
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&e->ev_replayed, hipEventDisableTiming);
+        if (he != hipSuccess) rc = fail(ISSPA_ERR_HIP, "md_engine_create: %s", hipGetErrorString(he));
     }
     // below a few thousand atoms the all-pairs kernel is a few microseconds and
     // the fork and join would cost more than the overlap saves
@@ -1216,14 +1236,7 @@ extern "C" int md_engine_capture(md_engine *e, int steps_per_graph, void *stream
     if (!e || steps_per_graph < 1) return fail(ISSPA_ERR_ARG, "md_engine_capture: bad argument");
     if (!stream) return fail(ISSPA_ERR_ARG, "md_engine_capture: needs a non-null stream");
     hipStream_t s = S(stream);
-    if (e->exec) {
-        (void)hipGraphExecDestroy(e->exec);
-        e->exec = nullptr;
-    }
-    if (e->graph) {
-        (void)hipGraphDestroy(e->graph);
-        e->graph = nullptr;
-    }
+    CHK(engine_drop_graph(e));
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = ISSPA_OK;
     for (int k = 0; k < steps_per_graph && rc == ISSPA_OK; ++k) rc = engine_step(e, s);
@@ -1247,5 +1260,9 @@ extern "C" int md_engine_replay(md_engine *e, int replays, void *stream)
     if (e->isspa && e->isspa->generation != e->captured_generation)
         return fail(ISSPA_ERR_STATE, "md_engine_replay: the IS-SPA tiling changed after capture; capture again");
     for (int r = 0; r < replays; ++r) HIPCHK(hipGraphLaunch(e->exec, S(stream)));
+    if (replays > 0) {
+        HIPCHK(hipEventRecord(e->ev_replayed, S(stream)));
+        e->replay_pending = true;
+    }
     return ISSPA_OK;
 }

@@ -89,6 +89,8 @@ inline float4 *rec_aux(isspa_model *m)
 FieldArgs field_args(isspa_model *m, const float4 *x);
 EpilogueArgs epilogue_args(isspa_model *m, bool inspect);
 ForceArgs force_args(isspa_model *m, const float4 *x);
+ReduceArgs reduce_args(isspa_model *m, const float4 *x, float4 *f, double *flj, double *fpol,
+                       double *fcdd);
 // the all-pairs term for positions x: split -> three FP64 component outputs;
 // otherwise f (float4, +=) or, with f null, o0 (double[N*3], +=)
 int nonbond_run(nonbond_model *m, const float *xyzq4_d, bool split, float4 *f, double *o0,

@@ -37,6 +37,7 @@ class atom
     float2 *lj_h = nullptr;                         // {A, B}
     int seed = 0;                                   // Philox key of the Andersen draws
     unsigned step = 0;                              // Philox counter, +1 per leapfrog_cuda
+    unsigned posRstStep = 0, velRstStep = 0;        // steps completed, from restart files
     float lbox = 0.0f;                              // set by nonbond_force_cuda_grid_block
     nonbond_model *nonbond = nullptr;               // built by the first nonbond_force_cuda
     // launch geometry kept for interface parity (the engine chooses its own)
@@ -58,7 +59,8 @@ class atom
     void print_vel();
     void print_for();
     void print_isspaf();
-    void write_rst_files(const char *posRstFileName, const char *velRstFileName, float lbox);
+    void write_rst_files(const char *posRstFileName, const char *velRstFileName, float lbox,
+                         unsigned step);
     void free_arrays();
     void free_arrays_gpu();
 };

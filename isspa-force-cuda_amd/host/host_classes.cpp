@@ -75,8 +75,53 @@ void atom::initialize_velocities(float T)
     }
 }
 
+// Exact restart.  The AMBER text restart holds 7 decimals per value
+// (src/atom_class.cpp:292-327), which does not round-trip a float velocity of
+// order 0.1.  Beside each restart file the writer keeps "<file>.exact": a
+// 16-byte header {"ISSPARST", n, step} and the n x 3 float values as written.
+// A resumed run (velRst = 1) reads them when they agree with the text values,
+// and continues the Philox counter at `step`, so it repeats bit for bit what
+// the uninterrupted run would have done.  Without the file the text values and
+// the step in the title line (if any) are used.
+struct ExactHeader {
+    char magic[8];
+    int32_t n;
+    uint32_t step;
+};
+static const char kExactMagic[8] = {'I', 'S', 'S', 'P', 'A', 'R', 'S', 'T'};
+
+static void exact_path(const char *path, char *out, size_t len)
+{
+    snprintf(out, len, "%s.exact", path);
+}
+
+// text values replaced by the exact ones when the side file belongs to them
+static bool read_exact(const char *path, int nAtoms, std::vector<float> &vals, unsigned *step)
+{
+    char ep[MAXCHAR + 8];
+    exact_path(path, ep, sizeof ep);
+    FILE *fh = fopen(ep, "rb");
+    if (!fh) return false;
+    ExactHeader h{};
+    std::vector<float> ex(3 * (size_t)nAtoms);
+    const bool ok = fread(&h, sizeof h, 1, fh) == 1 && !memcmp(h.magic, kExactMagic, 8) &&
+                    h.n == nAtoms && fread(ex.data(), sizeof(float), ex.size(), fh) == ex.size();
+    fclose(fh);
+    if (!ok) return false;
+    for (size_t k = 0; k < ex.size(); ++k)
+        if (!(fabsf(ex[k] - vals[k]) <= 1e-6f * std::max(1.0f, fabsf(ex[k])))) {
+            fprintf(stderr, "%s does not match %s (value %zu: %g vs %g); using the text values\n",
+                    ep, path, k, ex[k], vals[k]);
+            return false;
+        }
+    vals = ex;
+    *step = h.step;
+    return true;
+}
+
 // AMBER restart: two header lines, then 6 fields of width 12 per line
-static int read_rst7_into(const char *path, int nAtoms, float4 *dst, const char *what)
+static int read_rst7_into(const char *path, int nAtoms, float4 *dst, const char *what,
+                          unsigned *step)
 {
     FILE *fh = fopen(path, "r");
     if (!fh) {
@@ -86,8 +131,12 @@ static int read_rst7_into(const char *path, int nAtoms, float4 *dst, const char 
     char line[MAXCHAR];
     std::vector<float> vals;
     int header = 0;
+    *step = 0;
     while (fgets(line, MAXCHAR, fh) && (int)vals.size() < 3 * nAtoms) {
         if (header < 2) {
+            // this writer's title ends in "step <n>"
+            const char *at = header == 0 ? strstr(line, " step ") : nullptr;
+            if (at) *step = (unsigned)strtoul(at + 6, nullptr, 10);
             ++header;
             continue;
         }
@@ -105,6 +154,7 @@ static int read_rst7_into(const char *path, int nAtoms, float4 *dst, const char 
                 vals.size());
         return 2;
     }
+    read_exact(path, nAtoms, vals, step);
     for (int i = 0; i < nAtoms; ++i) {
         dst[i].x = vals[3 * i];
         dst[i].y = vals[3 * i + 1];
@@ -115,12 +165,12 @@ static int read_rst7_into(const char *path, int nAtoms, float4 *dst, const char 
 
 void atom::read_initial_positions(const char *f)
 {
-    if (read_rst7_into(f, nAtoms, pos_h, "coordinate")) exit(1);
+    if (read_rst7_into(f, nAtoms, pos_h, "coordinate", &posRstStep)) exit(1);
 }
 
 void atom::read_initial_velocities(const char *f)
 {
-    if (read_rst7_into(f, nAtoms, vel_h, "velocity")) exit(2);
+    if (read_rst7_into(f, nAtoms, vel_h, "velocity", &velRstStep)) exit(2);
 }
 
 void atom::open_traj_files(const char *forName, const char *posName, const char *velName)
@@ -188,11 +238,12 @@ void atom::print_isspaf()
     fflush(IFFile);
 }
 
-static void write_rst(const char *path, const char *title, int n, const float4 *v, float lbox)
+static void write_rst(const char *path, const char *title, int n, const float4 *v, float lbox,
+                      unsigned step)
 {
     FILE *fh = fopen(path, "w");
     if (!fh) return;
-    fprintf(fh, "%s\n%d\n", title, n);
+    fprintf(fh, "%s step %u\n%d\n", title, step, n);
     for (int i = 0; i < n; i += 2) {
         fprintf(fh, "%12.7f%12.7f%12.7f", v[i].x, v[i].y, v[i].z);
         if (i + 1 < n) fprintf(fh, "%12.7f%12.7f%12.7f\n", v[i + 1].x, v[i + 1].y, v[i + 1].z);
@@ -200,14 +251,32 @@ static void write_rst(const char *path, const char *title, int n, const float4 *
     }
     fprintf(fh, "%12.7f%12.7f%12.7f%12.7f%12.7f%12.7f\n", lbox, lbox, lbox, 90.0, 90.0, 90.0);
     fclose(fh);
+    char ep[MAXCHAR + 8];
+    exact_path(path, ep, sizeof ep);
+    FILE *bh = fopen(ep, "wb");
+    if (!bh) return;
+    ExactHeader h{};
+    memcpy(h.magic, kExactMagic, 8);
+    h.n = n;
+    h.step = step;
+    std::vector<float> ex(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        ex[3 * i] = v[i].x;
+        ex[3 * i + 1] = v[i].y;
+        ex[3 * i + 2] = v[i].z;
+    }
+    fwrite(&h, sizeof h, 1, bh);
+    fwrite(ex.data(), sizeof(float), ex.size(), bh);
+    fclose(bh);
 }
 
-// src/atom_class.cpp:292-327
-void atom::write_rst_files(const char *posName, const char *velName, float lbox)
+// src/atom_class.cpp:292-327; `step` = steps completed (the Philox step of
+// the next one)
+void atom::write_rst_files(const char *posName, const char *velName, float lbox, unsigned step)
 {
     get_pos_vel_from_gpu();
-    write_rst(posName, "Restart Positions from IS-SPA MI355X engine", nAtoms, pos_h, lbox);
-    write_rst(velName, "Restart Velocities from IS-SPA MI355X engine", nAtoms, vel_h, lbox);
+    write_rst(posName, "Restart Positions from IS-SPA MI355X engine", nAtoms, pos_h, lbox, step);
+    write_rst(velName, "Restart Velocities from IS-SPA MI355X engine", nAtoms, vel_h, lbox, step);
 }
 
 void atom::free_arrays()

@@ -62,8 +62,18 @@ int main(int argc, char *argv[])
     }
 
     atoms.read_initial_positions(configs.inputCoordFileName);
-    if (configs.velRst == 1) atoms.read_initial_velocities(configs.inputVelFileName);
-    else {
+    // a resumed run (velRst = 1) continues the Philox counter of the run that
+    // wrote the restart, so the MC points and Andersen draws of its steps are
+    // those the uninterrupted run would have used (with the same seed)
+    unsigned start = 0;
+    if (configs.velRst == 1) {
+        atoms.read_initial_velocities(configs.inputVelFileName);
+        start = atoms.velRstStep;
+        if (atoms.posRstStep != start)
+            fprintf(stderr, "warning: position restart at step %u, velocity restart at step %u; "
+                            "continuing from step %u\n", atoms.posRstStep, start, start);
+        printf("Resuming at step %u\n", start);
+    } else {
         srand((unsigned)configs.seed);
         atoms.initialize_velocities(configs.T);
     }
@@ -78,6 +88,7 @@ int main(int argc, char *argv[])
     dih_force_cuda_grid_block(dihs.nDihs, &dihs.gridSize, &dihs.blockSize, &dihs.minGridSize);
     isspas.initialize_gpu(atoms.nAtoms, configs.seed);
     isspa_grid_block(atoms.nAtoms, atoms.nPairs, configs.lbox, isspas);
+    isspas.step = atoms.step = start;
     if (configs.us == 1) {
         bias.initialize_gpu();
         us_grid_block(bias);
@@ -119,6 +130,7 @@ int main(int argc, char *argv[])
                                      configs.lbox, (uint64_t)(uint32_t)configs.seed, 0u, &eng),
                     "md_engine_create");
         isspa_check(isspa_stream_create(&stream), "isspa_stream_create");
+        isspa_check(md_engine_set_step(eng, start, stream), "md_engine_set_step");
         // several steps per graph replay when they divide the write interval,
         // so every write still falls on a replay boundary
         for (int k : {10, 5, 2})
@@ -179,7 +191,8 @@ int main(int argc, char *argv[])
     if (graph) isspa_check(isspa_stream_sync(stream), "sync");
 
     times.startWriteTimer();
-    atoms.write_rst_files(configs.posRstFileName, configs.velRstFileName, configs.lbox);
+    atoms.write_rst_files(configs.posRstFileName, configs.velRstFileName, configs.lbox,
+                          start + (unsigned)configs.nSteps);
     times.stopWriteTimer();
     times.print_final(configs.nSteps * configs.dtPs * 1.0e-3f);
 

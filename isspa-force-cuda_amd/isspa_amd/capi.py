@@ -21,6 +21,17 @@ class IsspaError(RuntimeError):
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
 
+# One HIP runtime per process: PyTorch's ROCm wheel carries its own
+# libamdhip64.so.7 and loads it by path.  Imported first, it is the copy the
+# library's libamdhip64.so.7 dependency resolves to; loaded after the library,
+# it would bring up a second runtime beside the system one, and the library's
+# own calls then fail ("no ROCm-capable device").  Without PyTorch the system
+# runtime is the only one.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _lib = C.CDLL(LIB_PATH)
 
 vp = C.c_void_p

@@ -300,6 +300,19 @@ def maxwell_velocities(system: System, seed: int) -> None:
 WORKLOAD_NAMES = ("chain22_nmc10", "chains50_1100atoms_nmc20", "chains909_19998atoms_nmc20")
 
 
+_LATTICE_SEEDS = {1: 11, 2: 12}
+
+
+def workload_seeds(index: int) -> dict:
+    """Every seed the plain workload `index` is built from: the chain
+    generator, the lattice placement (none for the single chain), the Philox
+    key of MC points and Andersen draws, and the Maxwell velocities."""
+    out = {"chain": SPEC["chain_seed"], "philox": 12345 + index, "velocities": 100 + index}
+    if index in _LATTICE_SEEDS:
+        out["lattice"] = _LATTICE_SEEDS[index]
+    return out
+
+
 def workload(index: int) -> System:
     """The plain workloads, by index:
     0: one chain, nMC=10, box 200 A;
@@ -311,15 +324,16 @@ def workload(index: int) -> System:
         s.lbox, s.n_mc = 200.0, 10
         s.xyzq[:, :3] += np.float32(100.0)
     elif index == 1:
-        s = make_lattice(50, 60.0, 4, 200.0, seed=11)
+        s = make_lattice(50, 60.0, 4, 200.0, seed=_LATTICE_SEEDS[1])
         s.n_mc = 20
     elif index == 2:
-        s = make_lattice(909, 160.0, 10, 400.0, seed=12)
+        s = make_lattice(909, 160.0, 10, 400.0, seed=_LATTICE_SEEDS[2])
         s.n_mc = 20
     else:
         raise ValueError(f"no plain workload {index}")
-    s.temperature_k, s.pnu, s.seed = 298.0, 0.01, 12345 + index
-    maxwell_velocities(s, seed=100 + index)
+    seeds = workload_seeds(index)
+    s.temperature_k, s.pnu, s.seed = 298.0, 0.01, seeds["philox"]
+    maxwell_velocities(s, seed=seeds["velocities"])
     s.name = WORKLOAD_NAMES[index]
     s.check()
     return s
