@@ -101,57 +101,82 @@ hipError_t launch_chunk_reduce3(const WinDesc *wins, const int *atom_win, const 
 // Slot layout: bonds [0, 2nb), angles [2nb, 2nb+3na), dihedrals 6 per term
 // (atoms i, j, k, l, then the 1-4 pair's i and l).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float3 mi3(float4 a, float4 b, float L, float h)
+// The bonded terms are evaluated in FP64.  Near equilibrium the reference's
+// FP32 forms cancel: 2k (r0/|r| - 1) loses the digits of r0/|r| that agree
+// with 1, and the angle and dihedral denominators lose theirs for nearly
+// straight angles, so one bond's FP32 error is ~1e-5 of a force of tens of
+// kcal/mol/A that mostly cancels in the atom's total.  These terms are O(N)
+// and cost microseconds; in FP64 every contribution is accurate before it is
+// stored as FP32.
+//
+// separation a - b, exact in FP64; the image is decided on the FP32
+// difference, as every other kernel decides it
+__device__ __forceinline__ double3 mi3d(float4 a, float4 b, float L, float h)
 {
-    return make_float3(min_image1(a.x - b.x, L, h), min_image1(a.y - b.y, L, h),
-                       min_image1(a.z - b.z, L, h));
+    const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const double kx = dx > h ? 1.0 : (dx < -h ? -1.0 : 0.0);
+    const double ky = dy > h ? 1.0 : (dy < -h ? -1.0 : 0.0);
+    const double kz = dz > h ? 1.0 : (dz < -h ? -1.0 : 0.0);
+    const double Ld = (double)L;
+    return make_double3((double)a.x - (double)b.x - kx * Ld, (double)a.y - (double)b.y - ky * Ld,
+                        (double)a.z - (double)b.z - kz * Ld);
 }
 
-// src/bond_force_cuda.cu:15-53
+__device__ __forceinline__ double dot3(double3 u, double3 v)
+{
+    return u.x * v.x + u.y * v.y + u.z * v.z;
+}
+
+__device__ __forceinline__ float4 f4_of(double s, double3 r)
+{
+    return make_float4((float)(s * r.x), (float)(s * r.y), (float)(s * r.z), 0.0f);
+}
+
+// src/bond_force_cuda.cu:15-53: F = 2k (r0/|r| - 1) r
 __device__ void bond_term(const BondedArgs &a, int q)
 {
     const int4 at = a.bond_atoms[q];
     const float2 pr = a.bond_params[at.z];
-    const float3 r = mi3(a.xyzq[at.x], a.xyzq[at.y], a.L, a.h);
-    const float d2 = r.x * r.x + r.y * r.y + r.z * r.z;
-    const float rinv = rsqrt_f32(d2);
-    const float fb = pr.x * (pr.y * rinv - 1.0f);
-    a.contrib[2 * q] = make_float4(fb * r.x, fb * r.y, fb * r.z, 0.0f);
-    a.contrib[2 * q + 1] = make_float4(-fb * r.x, -fb * r.y, -fb * r.z, 0.0f);
-    const float dd = d2 * rinv - pr.y;
-    a.energy[q] = 0.5 * (double)pr.x * (double)dd * (double)dd;
+    const double3 r = mi3d(a.xyzq[at.x], a.xyzq[at.y], a.L, a.h);
+    const double d = sqrt(dot3(r, r));
+    const double fb = (double)pr.x * ((double)pr.y / d - 1.0);
+    a.contrib[2 * q] = f4_of(fb, r);
+    a.contrib[2 * q + 1] = f4_of(-fb, r);
+    const double dd = d - (double)pr.y;
+    a.energy[q] = 0.5 * (double)pr.x * dd * dd;
 }
 
-// src/angle_force_cuda.cu:12-64
+// src/angle_force_cuda.cu:12-64: theta = acos(-c12/sqrt(c11 c22)) (clamped),
+// fa = 2k (theta - theta0) / sqrt(c11 c22 - c12^2)
 __device__ void angle_term(const BondedArgs &a, int q)
 {
     const int4 at = a.angle_atoms[q];
-    const float3 r1 = mi3(a.xyzq[at.x], a.xyzq[at.y], a.L, a.h);
-    const float3 r2 = mi3(a.xyzq[at.y], a.xyzq[at.z], a.L, a.h);
-    const float c11 = r1.x * r1.x + r1.y * r1.y + r1.z * r1.z;
-    const float c22 = r2.x * r2.x + r2.y * r2.y + r2.z * r2.z;
-    const float c12 = r1.x * r2.x + r1.y * r2.y + r1.z * r2.z;
-    const float b = -c12 / sqrtf(c11 * c22);
-    float th;
-    if (b >= 1.0f) th = 1.0e-16f;
-    else if (b <= -1.0f) th = 3.1415926535f;
-    else th = acosf(b);
+    const double3 r1 = mi3d(a.xyzq[at.x], a.xyzq[at.y], a.L, a.h);
+    const double3 r2 = mi3d(a.xyzq[at.y], a.xyzq[at.z], a.L, a.h);
+    const double c11 = dot3(r1, r1), c22 = dot3(r2, r2), c12 = dot3(r1, r2);
+    const double b = -c12 / sqrt(c11 * c22);
+    double th;
+    if (b >= 1.0) th = 1.0e-16;
+    else if (b <= -1.0) th = (double)3.1415926535f;
+    else th = acos(b);
     const float2 pr = a.angle_params[at.w];
-    const float fa = pr.x * (th - pr.y) / sqrtf(c11 * c22 - c12 * c12);
-    const float a11 = c12 / c11, a22 = c12 / c22;
+    const double dth = th - (double)pr.y;
+    const double fa = (double)pr.x * dth / sqrt(c11 * c22 - c12 * c12);
+    const double a11 = c12 / c11, a22 = c12 / c22;
     const int base = 2 * a.nb + 3 * q;
-    a.contrib[base] = make_float4(fa * (a11 * r1.x - r2.x), fa * (a11 * r1.y - r2.y),
-                                  fa * (a11 * r1.z - r2.z), 0.0f);
-    a.contrib[base + 1] = make_float4(fa * ((1.0f + a22) * r2.x - (1.0f + a11) * r1.x),
-                                      fa * ((1.0f + a22) * r2.y - (1.0f + a11) * r1.y),
-                                      fa * ((1.0f + a22) * r2.z - (1.0f + a11) * r1.z), 0.0f);
-    a.contrib[base + 2] = make_float4(fa * (r1.x - a22 * r2.x), fa * (r1.y - a22 * r2.y),
-                                      fa * (r1.z - a22 * r2.z), 0.0f);
-    const double dth = (double)th - (double)pr.y;
+    a.contrib[base] = make_float4((float)(fa * (a11 * r1.x - r2.x)), (float)(fa * (a11 * r1.y - r2.y)),
+                                  (float)(fa * (a11 * r1.z - r2.z)), 0.0f);
+    a.contrib[base + 1] =
+        make_float4((float)(fa * ((1.0 + a22) * r2.x - (1.0 + a11) * r1.x)),
+                    (float)(fa * ((1.0 + a22) * r2.y - (1.0 + a11) * r1.y)),
+                    (float)(fa * ((1.0 + a22) * r2.z - (1.0 + a11) * r1.z)), 0.0f);
+    a.contrib[base + 2] = make_float4((float)(fa * (r1.x - a22 * r2.x)), (float)(fa * (r1.y - a22 * r2.y)),
+                                      (float)(fa * (r1.z - a22 * r2.z)), 0.0f);
     a.energy[a.nb + q] = 0.5 * (double)pr.x * dth * dth;
 }
 
-// src/dih_force_cuda.cu:13-136
+// src/dih_force_cuda.cu:13-136: n V sin(n phi - phase) c22 / (sin phi b) and
+// the scaled 1-4 pair
 __device__ void dih_term(const BondedArgs &a, int q)
 {
     int4 at = a.dih_atoms[q];
@@ -163,65 +188,62 @@ __device__ void dih_term(const BondedArgs &a, int q)
     if (at.z > 0 && at.w > 0) {  // scaled 1-4 (index 0 never qualifies, quirk Q20 mirrored)
         const float2 lj = a.lj[a.ityp[at.x] * a.T + a.ityp[at.w]];
         const float2 sc = a.sc14[ty];
-        const float3 r = mi3(p1, p4, a.L, a.h);
-        const float rm = r.x * r.x + r.y * r.y + r.z * r.z;
-        const float r6 = 1.0f / (rm * rm * rm);
-        const float fe = p1.w * p4.w / (rm * sqrtf(rm) * sc.x);
-        const float fv = r6 * (12.0f * lj.x * r6 - 6.0f * lj.y) / (sc.y * rm);
-        const float s = fe + fv;
-        a.contrib[base + 4] = make_float4(s * r.x, s * r.y, s * r.z, 0.0f);
-        a.contrib[base + 5] = make_float4(-s * r.x, -s * r.y, -s * r.z, 0.0f);
-        e14c = (double)p1.w * (double)p4.w / (sqrt((double)rm) * (double)sc.x);
+        const double3 r = mi3d(p1, p4, a.L, a.h);
+        const double rm = dot3(r, r);
+        const double r6 = 1.0 / (rm * rm * rm);
+        const double qq = (double)p1.w * (double)p4.w;
+        const double fe = qq / (rm * sqrt(rm) * (double)sc.x);
+        const double fv = r6 * (12.0 * (double)lj.x * r6 - 6.0 * (double)lj.y) / ((double)sc.y * rm);
+        a.contrib[base + 4] = f4_of(fe + fv, r);
+        a.contrib[base + 5] = f4_of(-(fe + fv), r);
+        e14c = qq / (sqrt(rm) * (double)sc.x);
         e14v = ((double)lj.x * r6 * r6 - (double)lj.y * r6) / (double)sc.y;
     } else {
         a.contrib[base + 4] = make_float4(0.f, 0.f, 0.f, 0.f);
         a.contrib[base + 5] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    const float3 r12 = mi3(p1, p2, a.L, a.h);
-    const float3 r23 = mi3(p2, p3, a.L, a.h);
-    const float3 r34 = mi3(p3, p4, a.L, a.h);
-    const float c11 = r12.x * r12.x + r12.y * r12.y + r12.z * r12.z;
-    const float c22 = r23.x * r23.x + r23.y * r23.y + r23.z * r23.z;
-    const float c12 = r12.x * r23.x + r12.y * r23.y + r12.z * r23.z;
-    const float c33 = r34.x * r34.x + r34.y * r34.y + r34.z * r34.z;
-    const float c23 = r23.x * r34.x + r23.y * r34.y + r23.z * r34.z;
-    const float c13 = r12.x * r34.x + r12.y * r34.y + r12.z * r34.z;
-    const float t1 = c13 * c22 - c12 * c23;
-    const float t2 = c11 * c23 - c12 * c13;
-    const float t3 = c12 * c12 - c11 * c22;
-    const float t4 = c22 * c33 - c23 * c23;
-    const float t5 = c13 * c23 - c12 * c33;
-    const float t6 = -t1;
-    const float b = sqrtf(-t3 * t4);
-    const float aa = t6 / b;
+    const double3 r12 = mi3d(p1, p2, a.L, a.h);
+    const double3 r23 = mi3d(p2, p3, a.L, a.h);
+    const double3 r34 = mi3d(p3, p4, a.L, a.h);
+    const double c11 = dot3(r12, r12), c22 = dot3(r23, r23), c12 = dot3(r12, r23);
+    const double c33 = dot3(r34, r34), c23 = dot3(r23, r34), c13 = dot3(r12, r34);
+    const double t1 = c13 * c22 - c12 * c23;
+    const double t2 = c11 * c23 - c12 * c13;
+    const double t3 = c12 * c12 - c11 * c22;
+    const double t4 = c22 * c33 - c23 * c23;
+    const double t5 = c13 * c23 - c12 * c33;
+    const double t6 = -t1;
+    const double b = sqrt(-t3 * t4);
+    const double aa = t6 / b;
     const float4 pr = a.dih_params[ty];
-    float fd = 0.0f, phi;
-    if (aa <= -1.0f) {
-        phi = 3.1415926535f;
-    } else if (aa >= 1.0f) {
-        phi = 0.0f;
+    const double pn = (double)pr.x, V = (double)pr.y, ph = (double)pr.z;
+    double fd = 0.0, phi;
+    if (aa <= -1.0) {
+        phi = (double)3.1415926535f;
+    } else if (aa >= 1.0) {
+        phi = 0.0;
     } else {
-        phi = acosf(aa);
-        fd = pr.x * pr.y * sinf(pr.x * phi - pr.z) * c22 / (sinf(phi) * b);
+        phi = acos(aa);
+        fd = pn * V * sin(pn * phi - ph) * c22 / (sin(phi) * b);
     }
-    const float g1 = fd / t3, g4 = -fd / t4;
-    const float3 f1 = make_float3(g1 * (t1 * r12.x + t2 * r23.x + t3 * r34.x),
-                                  g1 * (t1 * r12.y + t2 * r23.y + t3 * r34.y),
-                                  g1 * (t1 * r12.z + t2 * r23.z + t3 * r34.z));
-    const float3 f4 = make_float3(g4 * (t4 * r12.x + t5 * r23.x + t6 * r34.x),
-                                  g4 * (t4 * r12.y + t5 * r23.y + t6 * r34.y),
-                                  g4 * (t4 * r12.z + t5 * r23.z + t6 * r34.z));
-    const float a12 = c12 / c22, a23 = c23 / c22;
-    a.contrib[base] = make_float4(f1.x, f1.y, f1.z, 0.f);
-    a.contrib[base + 1] = make_float4(-(1.0f + a12) * f1.x + a23 * f4.x,
-                                      -(1.0f + a12) * f1.y + a23 * f4.y,
-                                      -(1.0f + a12) * f1.z + a23 * f4.z, 0.f);
-    a.contrib[base + 2] = make_float4(a12 * f1.x - (1.0f + a23) * f4.x,
-                                      a12 * f1.y - (1.0f + a23) * f4.y,
-                                      a12 * f1.z - (1.0f + a23) * f4.z, 0.f);
-    a.contrib[base + 3] = make_float4(f4.x, f4.y, f4.z, 0.f);
+    const double g1 = fd / t3, g4 = -fd / t4;
+    const double3 f1 = make_double3(g1 * (t1 * r12.x + t2 * r23.x + t3 * r34.x),
+                                    g1 * (t1 * r12.y + t2 * r23.y + t3 * r34.y),
+                                    g1 * (t1 * r12.z + t2 * r23.z + t3 * r34.z));
+    const double3 f4 = make_double3(g4 * (t4 * r12.x + t5 * r23.x + t6 * r34.x),
+                                    g4 * (t4 * r12.y + t5 * r23.y + t6 * r34.y),
+                                    g4 * (t4 * r12.z + t5 * r23.z + t6 * r34.z));
+    const double a12 = c12 / c22, a23 = c23 / c22;
+    a.contrib[base] = f4_of(1.0, f1);
+    a.contrib[base + 1] = make_float4((float)(-(1.0 + a12) * f1.x + a23 * f4.x),
+                                      (float)(-(1.0 + a12) * f1.y + a23 * f4.y),
+                                      (float)(-(1.0 + a12) * f1.z + a23 * f4.z), 0.f);
+    a.contrib[base + 2] = make_float4((float)(a12 * f1.x - (1.0 + a23) * f4.x),
+                                      (float)(a12 * f1.y - (1.0 + a23) * f4.y),
+                                      (float)(a12 * f1.z - (1.0 + a23) * f4.z), 0.f);
+    a.contrib[base + 3] = f4_of(1.0, f4);
     const size_t eb = (size_t)a.nb + a.na + 3 * (size_t)q;
-    a.energy[eb] = (double)pr.y * (1.0 + cos((double)pr.x * (double)phi - (double)pr.z));
+    a.energy[eb] = V * (1.0 + cos(pn * phi - ph));
     a.energy[eb + 1] = e14c;
     a.energy[eb + 2] = e14v;
 }

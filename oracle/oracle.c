@@ -84,6 +84,16 @@ static inline double mi1d(double r, double L, double h)
     if (r < -h) return r + L;
     return r;
 }
+/* a - b in FP64 with the image the device takes: the shift is decided on the
+ * FP32 difference of the FP32 coordinates against the FP32 half box, so a
+ * pair at exactly half a box (where the single-image rule is discontinuous)
+ * gets the same image on both sides; the separation itself is FP64. */
+static inline double mi_dev(double a, double b, double L)
+{
+    const float rf = (float)a - (float)b, hf = 0.5f * (float)L;
+    const double k = rf > hf ? 1.0 : (rf < -hf ? -1.0 : 0.0);
+    return a - b - k * L;
+}
 
 /* FP32 pair separation and squared distance exactly as the device forms them. */
 static inline float sep_f32(const float *a, const float *b, float L, float h, float r[3])
@@ -339,7 +349,7 @@ void orc_nonbond(int n, const double *xyzq, int n_lj_types, const int *ityp, con
             if (j == i) continue;
             double r[3];
             for (int a = 0; a < 3; ++a)
-                r[a] = mi1d((double)xyzq[4 * i + a] - (double)xyzq[4 * j + a], L, h);
+                r[a] = mi_dev((double)xyzq[4 * i + a], (double)xyzq[4 * j + a], L);
             double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
             double d = sqrt(d2);
             double qq = qi * (double)xyzq[4 * j + 3];
@@ -383,8 +393,8 @@ void orc_nonbond(int n, const double *xyzq, int n_lj_types, const int *ityp, con
 /* ------------------------------------------------------------------------ */
 static void sep_d(const double *xyzq, int a, int b, double L, double h, double r[3])
 {
-    for (int k = 0; k < 3; ++k)
-        r[k] = mi1d((double)xyzq[4 * a + k] - (double)xyzq[4 * b + k], L, h);
+    (void)h;
+    for (int k = 0; k < 3; ++k) r[k] = mi_dev((double)xyzq[4 * a + k], (double)xyzq[4 * b + k], L);
 }
 
 /* src/bond_force_cuda.cu:15-53: F = 2k (r0/|r| - 1) r, E = k (|r| - r0)^2 */
