@@ -131,6 +131,16 @@ int isspa_model_destroy(isspa_model *m);
  * replaced by isspa_model_force_components. */
 int isspa_model_force(isspa_model *m, const float *xyzq4_d, float *f4_d, uint32_t step,
                       const uint32_t *step_d, void *stream);
+/* The same as isspa_model_force, allocating nothing: its per-call buffers
+ * come from a static device array of ISSPA_NOALLOC_SCRATCH_BYTES per GPU
+ * (held until the process ends).  No counterpart in the reference: for
+ * callers that cannot allocate device memory (stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  ISSPA_ERR_ARG if the model's size and
+ * tiling need more scratch than the array holds (see the message).  All calls
+ * on one GPU share the array: calls that overlap on one GPU race. */
+#define ISSPA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int isspa_model_force_noalloc(isspa_model *m, const float *xyzq4_d, float *f4_d, uint32_t step,
+                              const uint32_t *step_d, void *stream);
 /* the same, and the IS-SPA force alone also stored (overwritten, w = 0) in
  * f_isspa4_d by the same reduction kernel. Fills the isspaf_d argument of
  * src/isspa_force_cuda.cu:331-360, which the reference zeroes

@@ -90,6 +90,7 @@ _SIGS = {
                                   C.POINTER(vp)], i32),
     "isspa_model_destroy": ([vp], i32),
     "isspa_model_force": ([vp, vp, vp, u32, vp, vp], i32),
+    "isspa_model_force_noalloc": ([vp, vp, vp, u32, vp, vp], i32),
     "isspa_model_force_keep": ([vp, vp, vp, vp, u32, vp, vp], i32),
     "isspa_model_force_components": ([vp, vp, vp, vp, vp, u32, vp], i32),
     "isspa_model_get_field": ([vp, vp, vp, vp, vp, vp, vp], i32),
