@@ -140,7 +140,7 @@ def batched_bench(steps, plain_rounds, warmup, tune_reps, per_graph=1):
     W = len(windows)
     g = GpuSystem(merged, "cuda:0")
     tr = autotune.tune(g, reps=tune_reps, waves=(2048, 4096, 8192, 16384, 32768, 65536, 131072),
-                       log=log, confirm=autotune.engine_forces_ms)
+                       log=log, confirm=autotune.engine_step_ms)
     log(f"batched autotune: {tr.kind} {tr.value} -> {tr.ms:.4f} ms per {tr.measure}")
     K = max(1, int(per_graph))
     steps = max(K, steps - steps % K)
@@ -281,6 +281,9 @@ def main():
     ap.add_argument("--target-waves", type=int, default=0, help="automatic tiling at this target")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run an untimed copy of this many MiB (counter calibration)")
+    ap.add_argument("--traffic-sweep", default="",
+                    help="after the timed steps, one untimed launch of each IS-SPA stage at each "
+                         "of these force chunk counts (comma-separated; counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
@@ -315,9 +318,9 @@ def main():
         g.set_target_waves(args.target_waves)
     elif not args.no_tune:
         # candidates ranked by the IS-SPA pass, the leaders confirmed on the
-        # engine's whole force evaluation (its all-pairs kernel overlaps the
-        # IS-SPA passes)
-        tune = autotune.tune(g, reps=2, log=log, confirm=autotune.engine_forces_ms)
+        # graph-replayed MD step that is timed below (its all-pairs kernel
+        # overlaps the IS-SPA passes)
+        tune = autotune.tune(g, reps=3, log=log, confirm=autotune.engine_step_ms)
         log(f"autotune: {tune.kind} {tune.value} -> {tune.ms:.4f} ms per {tune.measure}")
     splits, chunks = g.tiling()
     if args.copy_mib:
@@ -355,6 +358,16 @@ def main():
 
     prof = g.profile(args.profile_reps)
     log("stage ms: " + ", ".join(f"{k} {v:.4f}" for k, v in prof.items()))
+    if args.traffic_sweep:
+        # k_force launches at other chunk counts, told apart in a counter pass
+        # by their grid (chunks x receiver blocks x 256 work-items)
+        rblocks = (s.n_atoms + 255) // 256
+        for c in (int(v) for v in args.traffic_sweep.split(",")):
+            g.set_tiling(splits, c)
+            _, c_eff = g.tiling()
+            g.profile(1)
+            log(f"traffic sweep: {c_eff} force chunks, k_force grid {c_eff * rblocks * 256}")
+        g.set_tiling(splits, chunks)
     N, P = s.n_atoms, s.n_points
     force_flop = FLOP_FORCE_PAIR * N * P
     achieved = force_flop / (prof["force"] * 1e-3)
@@ -402,20 +415,43 @@ def main():
                                      "ms": round(tune.ms, 4), "measure": tune.measure,
                                      "candidates": len(tune.table)}
     # HBM-side bytes per k_force launch from the committed FETCH_SIZE /
-    # WRITE_SIZE passes (copy-calibrated).  k_force's traffic depends on the
-    # force chunk count only (it writes 6 FP64 partials per chunk and atom), so
-    # the figure is reported when this run's chunk count is the profiled one
+    # WRITE_SIZE passes (copy-calibrated) at several force chunk counts.
+    # k_force reads the point records and writes 6 FP64 partials per (chunk,
+    # atom): its WRITE_SIZE is linear in the chunk count and its FETCH_SIZE
+    # nearly constant, so both are interpolated linearly between the two
+    # profiled counts around this run's; outside the profiled range: null
     tr_path = os.path.join(ROOT, "profiles", "k_force_19998atoms_nmc20_traffic.json")
     if args.workload == 2 and os.path.exists(tr_path):
         with open(tr_path) as fh:
             tr = json.load(fh)
-        same = int(tr["tiling"][1]) == chunks
-        out["roofline"]["traffic"] = tr["traffic_bytes"] if same else None
-        out["roofline"]["traffic_tiling"] = tr["tiling"]
+        pts = sorted(tr["points"], key=lambda q: q["chunks"])
+        traffic, note = None, (f"outside the profiled chunk counts "
+                               f"{pts[0]['chunks']}-{pts[-1]['chunks']}; this run used {chunks}")
+        for lo, hi in zip(pts, pts[1:]):
+            if lo["chunks"] <= chunks <= hi["chunks"]:
+                t = (chunks - lo["chunks"]) / max(1, hi["chunks"] - lo["chunks"])
+                kib = lambda k: lo[k] + t * (hi[k] - lo[k])
+                traffic = int(round((kib("fetch_kib") * tr["fetch_scale"] +
+                                     kib("write_kib") * tr["write_scale"]) * 1024))
+                note = (f"FETCH_SIZE/WRITE_SIZE interpolated between the profiled {lo['chunks']} and "
+                        f"{hi['chunks']} force chunks (this run {chunks})")
+                break
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_note"] = note
         out["roofline"]["traffic_source"] = tr["summary"]
-        if not same:
-            out["roofline"]["traffic_note"] = (f"profiled at {tr['tiling'][1]} force chunks; "
-                                               f"this run used {chunks}")
+    # the VALU-instruction counter pass of the same kernels (committed)
+    vc_path = os.path.join(ROOT, "profiles", "isspa_passes_19998atoms_nmc20_valu_counters.json")
+    if args.workload == 2 and os.path.exists(vc_path):
+        with open(vc_path) as fh:
+            vc = json.load(fh)
+        out["roofline"]["counter_check"] = {
+            k: {"valu_per_wave_pair": vc[k]["valu_per_wave_pair"],
+                "executed_flop_per_pair": vc[k]["executed_flop_per_pair"],
+                "algorithmic_flop_per_pair": vc[k]["algorithmic_flop_per_pair"],
+                "executed_over_algorithmic": vc[k]["executed_over_algorithmic"]}
+            for k in ("k_force", "k_field")}
+        out["roofline"]["counter_check"]["source"] = vc["summary"]
+
     g.close()
     if not args.no_batch:
         out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,

@@ -345,6 +345,9 @@ ForceArgs force_args(isspa_model *m, const float4 *x)
     fa.aux4 = rec_aux(m);
     fa.xyzq = x; fa.itype = m->itype; fa.rmax2 = m->rmax2; fa.tab = m->tv; fa.nmc = m->d.n_mc;
     fa.pbc_flag = m->pbc_flag; fa.part = m->part_f; fa.C = m->C;
+    // one system: every block has the same work, so each XCD may take a
+    // contiguous run of chunks (batches keep their longest-first order)
+    fa.xcd_runs = m->lay.W == 1;
     return fa;
 }
 

@@ -301,14 +301,15 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     const long long m = gp - (long long)W.a0 * a.nmc;
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
     int cnt = 0;
-    // two splits' loads in flight at a time, combined in split order
+    // kEpiBatch splits' loads in flight at a time, combined in split order
+    constexpr int kEpiBatch = 4;
     int sp = 0;
-    for (; sp + 1 < W.splits; sp += 2) {
-        float pg[2];
-        int pc[2];
-        double ps[2][3], pe[2][3];
+    for (; sp + kEpiBatch - 1 < W.splits; sp += kEpiBatch) {
+        float pg[kEpiBatch];
+        int pc[kEpiBatch];
+        double ps[kEpiBatch][3], pe[kEpiBatch][3];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < kEpiBatch; ++u) {
             const long long o = W.fpo + (long long)(sp + u) * P + m;
             pg[u] = a.part_g[o];
             pc[u] = a.part_cnt[o];
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
             }
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < kEpiBatch; ++u) {
             g *= (double)pg[u];
             cnt += pc[u];
 #pragma unroll
@@ -548,7 +549,7 @@ __device__ __forceinline__ void force_chunk(const ForceArgs &a, const float4 *__
 template <bool SPLIT, bool AN>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 {
-    const int4 bt = a.blocks[blockIdx.x];
+    const int4 bt = a.blocks[a.xcd_runs ? xcd_run_entry(blockIdx.x, gridDim.x) : blockIdx.x];
     const WinDesc W = a.wins[bt.x];
     const int il = bt.y * kBlock + threadIdx.x;  // receiver within the window
     const int chunk = bt.z;
@@ -562,6 +563,31 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     const float4 xi = a.xyzq[i];
     const int ti = a.itype[i];
     const float R2 = valid ? a.rmax2[ti] : -1.0f;
+    // Warm-up: the chunk's records and aux entries are requested at block
+    // start, one 128-byte line per work-item, the lines dealt over the
+    // chunk's receiver blocks, so that all of them are in flight at once.
+    // Right after the epilogue has written the records they sit in another
+    // XCD's L2 or in memory, and the loop's one-record-ahead scalar loads
+    // would otherwise wait for each line in turn.  The loaded values are only
+    // kept alive (an empty asm takes them after the loop), never used.
+    float w0 = 0.0f, w1 = 0.0f;
+    {
+        const int nrb = (W.n + kBlock - 1) / kBlock;  // receiver blocks of the chunk
+        const int np = m1 - m0;
+        const int nrl = (16 * np + 31) / 32;          // record lines, then aux lines
+        const int nl = nrl + (4 * np + 31) / 32;
+        const float *rb = reinterpret_cast<const float *>(rec + 4 * m0);
+        const float *ab = reinterpret_cast<const float *>(aux + m0);
+        const int l0 = bt.y + nrb * (int)threadIdx.x, l1 = l0 + nrb * kBlock;
+        if (l0 < nl) {
+            const float *p = l0 < nrl ? rb + 32 * l0 : ab + 32 * (l0 - nrl);
+            w0 = *p;
+        }
+        if (l1 < nl) {
+            const float *p = l1 < nrl ? rb + 32 * l1 : ab + 32 * (l1 - nrl);
+            w1 = *p;
+        }
+    }
     // the receiver type's F table for the in-sphere branch: LDS when it fits;
     // the analytic model keeps its four F constants in registers
     extern __shared__ float lds_tab[];
@@ -585,6 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
         force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2,
                                AN ? nullptr : a.tab.f + ti * a.tab.ld_f, fp, acc);
     }
+    asm volatile("" ::"v"(w0), "v"(w1));
     if (!valid) return;
     const long long o = W.ppo + (long long)chunk * W.n + il;
     const long long C = a.C;

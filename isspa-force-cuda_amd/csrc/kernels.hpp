@@ -128,7 +128,18 @@ struct ForceArgs {
     const int *pbc_flag;
     double *part;         // [comps][C] (S_e, S_lj) or (pol, lj, cdd)
     long long C;
+    int xcd_runs;         // 1: each XCD takes a contiguous run of the table (equal work items)
 };
+
+// Workgroup b of a launch goes to XCD b % 8 (round-robin).  With xcd_runs the
+// table entry of workgroup b is the (b / 8)-th of XCD b % 8's contiguous run,
+// so the receiver blocks of one chunk share an L2 instead of being dealt to
+// all eight.  A bijection on [0, nb) for any nb.
+__device__ __forceinline__ int xcd_run_entry(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
 
 struct ReduceArgs {
     const WinDesc *wins;

@@ -17,24 +17,28 @@ namespace isspa {
 constexpr int kSumAtoms = 32;
 constexpr int kSumSlices = 8;
 
+// Chunks whose partial loads one slice keeps in flight at a time; the
+// batch only groups the loads, the adds stay in chunk order.
+constexpr int kSumBatch = 8;
+
 // IS-SPA force partials of atom i (COMPS = 6, or 9 in split mode), chunks of
-// slice sl; four chunks' loads in flight at a time, added in chunk order.
+// slice sl, added in chunk order.
 template <int COMPS>
 __device__ inline void isspa_slice_sum(const ReduceArgs &a, int i, int sl, double (&s)[COMPS])
 {
     const WinDesc W = a.wins[a.atom_win[i]];
     const long long base = W.ppo + (i - W.a0);
     int c = sl;
-    for (; c + 3 * kSumSlices < W.chunks; c += 4 * kSumSlices) {
-        double v[4][COMPS];
+    for (; c + (kSumBatch - 1) * kSumSlices < W.chunks; c += kSumBatch * kSumSlices) {
+        double v[kSumBatch][COMPS];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kSumBatch; ++u) {
             const long long o = base + (long long)(c + u * kSumSlices) * W.n;
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) v[u][k] = a.part[k * a.C + o];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kSumBatch; ++u)
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
     }
@@ -69,7 +73,7 @@ __device__ inline void bonded_slice_sum(const float4 *__restrict__ contrib,
 }
 
 // all-pairs partials of atom i (COMPS = 3, or 9 in split mode), partner
-// chunks of slice sl; four chunks' loads in flight at a time, added in order
+// chunks of slice sl, added in chunk order
 template <int COMPS>
 __device__ inline void nonbond_slice_sum(const WinDesc *__restrict__ wins,
                                          const int *__restrict__ atom_win,
@@ -79,16 +83,16 @@ __device__ inline void nonbond_slice_sum(const WinDesc *__restrict__ wins,
     const WinDesc W = wins[atom_win[i]];
     const long long base = W.npo + (i - W.a0);
     int c = sl;
-    for (; c + 3 * kSumSlices < W.nbc; c += 4 * kSumSlices) {
-        double v[4][COMPS];
+    for (; c + (kSumBatch - 1) * kSumSlices < W.nbc; c += kSumBatch * kSumSlices) {
+        double v[kSumBatch][COMPS];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kSumBatch; ++u) {
             const long long o = base + (long long)(c + u * kSumSlices) * W.n;
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) v[u][k] = part[k * B + o];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kSumBatch; ++u)
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
     }

@@ -132,6 +132,62 @@ def fit(model, n_frames: int, theta0, q0, target, steps: int = 200, lr: float = 
         return (th_base * (1.0 + rel)).cpu().numpy(), (q_base + qs * dq).cpu().numpy(), hist
 
 
+def curvature(base, frames, theta, q, fd_rel_h: float = 1e-2, device: str = "cuda:0"):
+    """Gauss-Newton curvature of the force-matching loss along the per-type
+    parameters, in relative units (theta_t,k -> theta_t,k (1 + delta)).
+
+    For each frame (one DiffForces model of that frame alone, its MC points
+    drawn at system id 0, fixed through the analysis) every row of the Jacobian
+    J = d F / d delta comes from the hand-written adjoint with a unit
+    cotangent (3N adjoint calls reusing one forward).  The curvature of
+    L = mean |F - F_t|^2 / mean |F_t|^2 at the optimum is
+    H = 2 J^T J / (n mean |F_t|^2) over the n force components of all frames.
+    Central differences of the forward (relative step fd_rel_h) check the
+    columns of J.  Returns H, its eigen-decomposition and the check."""
+    import dataclasses
+
+    import torch
+
+    from .diff import DiffForces, DiffParams
+    prm = DiffParams.from_system(base)
+    th0 = torch.as_tensor(theta, dtype=torch.float64, device=device)
+    qq = torch.as_tensor(q, dtype=torch.float64, device=device)
+    T, K = th0.shape
+    n = 3 * base.n_atoms
+    JtJ = torch.zeros((T * K, T * K), dtype=torch.float64, device=device)
+    fsq, fd_err = 0.0, []
+    for w, x in enumerate(frames):
+        c = dataclasses.replace(base)
+        c.xyzq = base.xyzq.copy()
+        c.xyzq[:, :3] = x
+        m = DiffForces(c, prm, device, step=0)
+        fsq += float((m.forces(qq, th0) ** 2).sum())
+        rows = torch.empty((n, T * K), dtype=torch.float64, device=device)
+        lam = torch.zeros((base.n_atoms, 3), dtype=torch.float64, device=device)
+        for k in range(n):
+            lam.zero_()
+            lam.view(-1)[k] = 1.0
+            _, gt = m.vjp(qq, th0, lam, after_forces=True)
+            rows[k] = (gt * th0).reshape(-1)
+        JtJ += rows.T @ rows
+        if w == 0:
+            # central differences of the forward against the adjoint's columns
+            for a in range(T * K):
+                d = torch.zeros_like(th0)
+                d.view(-1)[a] = fd_rel_h * th0.view(-1)[a]
+                col = ((m.forces(qq, th0 + d) - m.forces(qq, th0 - d)) / (2 * fd_rel_h)).reshape(-1)
+                ref = rows[:, a]
+                fd_err.append(float((col - ref).norm() / max(float(ref.norm()), 1e-300)))
+        m.close()
+    ncomp = n * len(frames)
+    norm = fsq / ncomp
+    H = 2.0 * JtJ / (ncomp * norm)
+    w_, V = torch.linalg.eigh(H)
+    return {"H": H.cpu().numpy(), "eigenvalues": w_.cpu().numpy(),
+            "eigenvectors": V.cpu().numpy(), "fd_column_rel": fd_err, "force_norm": norm,
+            "components": ncomp}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--frames", type=int, default=256)
@@ -142,6 +198,8 @@ def main(argv=None):
     ap.add_argument("--lbfgs", action="store_true", help="L-BFGS instead of Adam (diagnostic)")
     ap.add_argument("--stride", type=int, default=20)
     ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--curvature", action="store_true",
+                    help="Gauss-Newton curvature along the per-type parameters instead of a fit")
     args = ap.parse_args(argv)
     import torch
 
@@ -162,6 +220,25 @@ def main(argv=None):
     dev = model.dev
     th_true = prm.theta
     q_true = prm.q
+    if args.curvature:
+        model.close()
+        c = curvature(base, frames, th_true, q_true)
+        w, V = c["eigenvalues"], c["eigenvectors"]
+        T = th_true.shape[0]
+        names = [f"{n}[{t}]" for t in range(T) for n in ("sigma", "eps", "x0", "g0", "alpha")]
+        diag = np.diag(c["H"])
+        per_kind = {n: [float(diag[t * 5 + k]) for t in range(T)]
+                    for k, n in enumerate(("sigma", "eps", "x0", "g0", "alpha"))}
+        modes = []
+        for i in np.argsort(w)[::-1]:
+            v = V[:, i]
+            top = np.argsort(-np.abs(v))[:3]
+            modes.append({"eigenvalue": float(w[i]),
+                          "largest": [(names[j], round(float(v[j]), 3)) for j in top]})
+        print(json.dumps({"workload": f"{args.frames} frames x {base.n_atoms} atoms, nMC={base.n_mc}",
+                          "curvature_diag": per_kind, "modes": modes,
+                          "fd_column_rel": c["fd_column_rel"]}), flush=True)
+        return
     target = model.forces(torch.as_tensor(np.tile(q_true, args.frames), device=dev),
                           torch.as_tensor(th_true, device=dev))
     rng = np.random.default_rng(args.seed)

@@ -80,9 +80,10 @@ def test_tune_picks_fastest_pass_and_confirm_overrides():
         calls.append(model.tiling())
         return 1.0 if model.tiling() == (8, 32) else 5.0
 
+    confirm.measure = "engine step"
     g2 = _FakeModel()
     r2 = autotune.tune(g2, reps=1, confirm=confirm)
-    assert r2.measure == "engine forces"
+    assert r2.measure == "engine step"
     assert len(calls) == 5 and (8, 32) in calls and calls.count((8, 64)) == 2
     assert r2.kind == "tiling" and r2.value == (8, 32)
     assert g2.tiling() == (8, 32) and r2.ms == 1.0

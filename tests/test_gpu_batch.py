@@ -159,6 +159,33 @@ def test_batch_graph_replay_matches_eager(merged):
     graph.close()
 
 
+def test_autotune_engine_step_restores_state(merged):
+    """Tuning with the graph-replayed step as the confirmation runs MD steps
+    on the system; positions, velocities and the step counter are restored
+    after, so a run that follows equals a run of an untuned copy at the chosen
+    geometry, bit for bit."""
+    from isspa_amd import autotune
+    tuned = _gpu(merged)
+    x0, v0 = tuned.x.clone(), tuned.v.clone()
+    r = autotune.tune(tuned, reps=1, waves=(2048, 8192), tilings=[(2, 3), (3, 5)],
+                      confirm=autotune.engine_step_ms)
+    print(f"case autotune by engine step: {r.kind} {r.value} {r.ms:.4f} ms", flush=True)
+    assert r.measure == "engine step" and np.isfinite(r.ms) and r.ms > 0
+    assert torch.equal(tuned.x, x0) and torch.equal(tuned.v, v0)
+    ref = _gpu(merged)
+    autotune._apply(ref, r.kind, r.value)
+    assert ref.tiling() == tuned.tiling()
+    for g in (tuned, ref):
+        g.set_step(0)
+        g.capture(1)
+        g.replay(3)
+    torch.cuda.synchronize()
+    assert np.array_equal(tuned.x.cpu().numpy(), ref.x.cpu().numpy())
+    assert np.array_equal(tuned.v.cpu().numpy(), ref.v.cpu().numpy())
+    tuned.close()
+    ref.close()
+
+
 def _mixed_box_windows():
     """Four windows whose boxes make the minimum image fire in two of them:
     a 22-atom chain in an 18 A box, 10 chains (220 atoms) in a 40 A box, and

@@ -90,10 +90,13 @@ def test_isspa_pbc_path(cfg0):
     g.close()
 
 
-def test_isspa_large_sampled():
-    """19,998 atoms, nMC=20 (the benchmark workload): all points bit for bit,
-    the field on 2,000 sampled points, and the force on 48 sampled receivers
-    computed by the oracle from the GPU's own field."""
+def test_isspa_large_full_field_and_sampled_forces():
+    """19,998 atoms, nMC=20 (the benchmark workload), against oracle-only
+    inputs: all 399,960 MC points bit for bit, the FP64 oracle field of every
+    point (8.0e9 point-atom pairs on the host cores) against the device's,
+    and the forces on 48 sampled receivers computed by the oracle from its own
+    points and field."""
+    import time
     from isspa_amd import synth
     s = synth.workload(2)
     g = _gpu(s, nonbond=False, bonded=False)
@@ -101,17 +104,19 @@ def test_isspa_large_sampled():
     fld = {k: v.cpu().numpy() for k, v in g.field().items()}
     ref_pts = po.mc_points(s, step=0)
     assert np.array_equal(fld["pts"][:, :3].view(np.uint32), ref_pts[:, :3].view(np.uint32))
-    rng = np.random.default_rng(0)
-    sel = np.sort(rng.choice(s.n_points, 2000, replace=False))
-    rf = po.isspa_field_subset(s, ref_pts[sel], sel // s.n_mc)
-    assert np.array_equal(fld["cnt"][sel], rf["cnt"])
+    t0 = time.perf_counter()
+    rf = po.isspa_field(s, ref_pts)
+    print(f"  oracle field of {s.n_points} points: {time.perf_counter() - t0:.1f} s", flush=True)
+    assert np.array_equal(fld["cnt"], rf["cnt"])
     for k in ("g", "E", "E0"):
-        assert max_abs_rel_rms(fld[k][sel], rf[k]) < 1e-6, k
+        err = max_abs_rel_rms(fld[k], rf[k])
+        print(f"  field {k}: rel RMS {err:.3e} over all points", flush=True)
+        assert err < 1e-6, k
+    rng = np.random.default_rng(0)
     atoms = np.sort(rng.choice(s.n_atoms, 48, replace=False))
-    field = {k: fld[k] for k in ("g", "cnt", "E", "E0")}
     refs = {k: [] for k in ("lj", "pol", "cdd")}
     for a in atoms:
-        ref = po.isspa_force(s, ref_pts, field, atoms=(int(a), int(a) + 1))
+        ref = po.isspa_force(s, ref_pts, rf, atoms=(int(a), int(a) + 1))
         for k in refs:
             refs[k].append(ref[k][a])
     for k in refs:
@@ -128,8 +133,9 @@ def test_isspa_mean_over_10k_steps_matches_reference_curand_frames(adi2):
     of timing_tests/ADI2/10000MC/compare/ISSPA_force_{LJsolv,Csolv,Csolv_CDD}_10000.xyz
     (mean and standard error per atom and axis in tests/golden/adi2_isspa_stats.npz).
     z = (mean_ours - mean_ref)/sqrt(se_ours^2 + se_ref^2).  Random halvings of
-    the fixture give z-RMS up to ~1.46, so the bar is z-RMS < 1.8 and
-    max |z| < 5 over the 132 values of each component."""
+    the fixture give z-RMS up to ~1.46, so the bar is z-RMS < 1.8; every one
+    of the 132 values of each component must lie within 3 sigma (the draws
+    are fixed by the seed, so the outcome is deterministic)."""
     import os
     import torch
     from conftest import GOLDEN
@@ -155,5 +161,5 @@ def test_isspa_mean_over_10k_steps_matches_reference_curand_frames(adi2):
         zrms = float(np.sqrt((z ** 2).mean()))
         print(f"{name}: z-RMS {zrms:.2f} max|z| {np.abs(z).max():.2f} "
               f"(|z| > 3 for {int((np.abs(z) > 3).sum())} of {z.size})", flush=True)
-        assert zrms < 1.8 and np.abs(z).max() < 5.0, name
+        assert zrms < 1.8 and np.abs(z).max() < 3.0, name
     g.close()
