@@ -4,8 +4,9 @@
 
 A step is one full MD step of the benchmark workload -- bonded terms, IS-SPA
 MC points + field + force passes, all-pairs nonbond, leap-frog + Andersen --
-replayed from a captured hipGraph with every input resident in HBM.  Prints
-ONE JSON line on stdout (diagnostics go to stderr).
+replayed from a captured hipGraph (10 steps per graph when 10 divides the
+warm-up and timed step counts, else 1) with every input resident in HBM.
+Prints ONE JSON line on stdout (diagnostics go to stderr).
 
 Roofline: the dominant kernel is the IS-SPA force pass.  Its algorithmic
 FLOPs per launch are 70 per (atom, point) pair (the counting convention of
@@ -17,15 +18,15 @@ kernel runs on (isspa_model_profile).  Peak: 157.3 TFLOP/s FP32 vector.
 Batched umbrella windows (reported beside the headline, "batched"): 64
 windows of uneven size (32 two-atom LJ pairs, 24 chains of 22 atoms, 8
 systems of 220 atoms; each with a k = 10 distance restraint, centres 3-15 A),
-nMC=20, run as ONE batched model whose whole step is one graph replay, against
-the plain per-window loop (64 single-window engines, one graph replay each per
-step, back to back on one stream).  Both report aggregate window-ns/day
+nMC=20, run as ONE batched model whose steps are graph replays, against the
+plain per-window loop (64 single-window engines, replayed back to back on one
+stream), both with the same number of steps per graph (--steps-per-graph).  Both report aggregate window-ns/day
 (sum over windows of ns/day at dt = 2 fs); the launch geometry of the batched
 model is chosen by isspa_amd.autotune.
 
 The reference's own published run ("reference_published_run"): ADI2, 44
 atoms, nMC=100, one distance restraint, the whole MD step on the engine at
-10 steps per graph replay, in ns/day beside the GTX 1080 figure of
+50 steps per graph replay, in ns/day beside the GTX 1080 figure of
 timing_tests/ADI2_US/window_6.5/ADI2.run0.log (also alone: --adi2-only).
 
 CPU baseline: the reference's own src/isspa_force.omp.c (built in place into
@@ -269,9 +270,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--plain-rounds", type=int, default=100,
                     help="rounds of the per-window loop in the batched comparison (0: skip it)")
-    ap.add_argument("--steps-per-graph", type=int, default=1,
-                    help="steps captured per graph in the batched comparison (both sides) "
-                         "and in --adi2-only")
+    ap.add_argument("--steps-per-graph", type=int, default=10,
+                    help="steps captured per graph: the headline step (when it divides --steps "
+                         "and --warmup, else 1), the batched comparison (both sides) and "
+                         "--adi2-only")
     ap.add_argument("--adi2-only", action="store_true",
                     help="run only the reference's published ADI2 configuration on the engine "
                          "and print its JSON object")
@@ -333,16 +335,21 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy: {nbytes} bytes read, {nbytes} bytes written")
         del src, dst
+    # K steps per graph replay when K divides both the warm-up and the timed
+    # steps, so exactly --steps steps are timed
+    K = args.steps_per_graph
+    if K < 1 or args.steps % K or args.warmup % K:
+        K = 1
     g.set_step(0)
-    g.capture(1)
-    g.replay(args.warmup)
+    g.capture(K)
+    g.replay(args.warmup // K)
     torch.cuda.synchronize()
     side = g._side()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     ev0.record(side)
-    g.replay(args.steps)
+    g.replay(args.steps // K)
     ev1.record(side)
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
@@ -389,9 +396,9 @@ def main():
         "config": {
             "workload": f"{s.name}: {N // 22} seeded copies of the 22-atom chain "
                         f"({N} atoms), nMC={s.n_mc}, box {s.lbox:g} A; bonded + IS-SPA + "
-                        "all-pairs nonbond + leap-frog/Andersen, one hipGraph replay per step",
+                        f"all-pairs nonbond + leap-frog/Andersen, hipGraph replays of {K} step(s)",
             "n_atoms": N, "n_mc": s.n_mc, "mc_points": P, "dt_fs": 2.0,
-            "field_splits": splits, "force_chunks": chunks,
+            "field_splits": splits, "force_chunks": chunks, "steps_per_graph": K,
             "seeds": synth.workload_seeds(args.workload),
         },
         "ns_per_day": round(0.1728 * steps_per_s, 4),
@@ -459,7 +466,7 @@ def main():
         # the one configuration the reference publishes timings for (ns/day on
         # a GTX 1080), beside the headline: a different metric, so it does not
         # set vs_baseline
-        out["reference_published_run"] = adi2_bench(args.adi2_steps, 200, 10)
+        out["reference_published_run"] = adi2_bench(args.adi2_steps, 200, 50)
     if not args.no_cpu:
         # the reference's own OpenMP file times 1..10 threads
         # (src/isspa_force.omp.c:172-199): here 1 thread (one call) and the

@@ -52,10 +52,25 @@ __host__ __device__ inline int diff_tab_floats(const TableView &t)
     return t.ntypes * (int)(sizeof(DiffType) / sizeof(float));
 }
 
-// copy n floats into LDS with the whole block (n is block-uniform)
+// copy n floats into LDS with the whole block (n is block-uniform); eight
+// loads per thread are issued before their stores, so a small system's
+// kernel waits for the table once rather than once per 256 floats
 __device__ __forceinline__ void stage_lds(float *dst, const float *__restrict__ src, int n)
 {
-    for (int t = threadIdx.x; t < n; t += kBlock) dst[t] = src[t];
+    constexpr int kU = 8;
+    for (int t0 = threadIdx.x; t0 < n; t0 += kU * kBlock) {
+        float v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int t = t0 + u * kBlock;
+            v[u] = t < n ? src[t] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int t = t0 + u * kBlock;
+            if (t < n) dst[t] = v[u];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -218,6 +233,16 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     const long long gp = (long long)W.a0 * a.nmc + m;
     const float4 p = valid ? a.pts[gp] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     const bool pbc = a.pbc_flag[bt.x] != 0;
+    // Warm-up: the 128-byte lines of the split's atoms (8 atoms a line, up to
+    // 256 lines) are requested at block start, one a work-item, so that the
+    // loop's scalar atom loads find them in L2 instead of waiting for each
+    // line in turn (the positions were last written by another XCD).  The
+    // value is only kept alive by the empty asm after the loop.
+    float w0 = 0.0f;
+    {
+        const int l = (j0 >> 3) + (int)threadIdx.x;
+        if (j1 > j0 && l <= ((j1 - 1) >> 3)) w0 = a.xyzq[8 * l].x;
+    }
     // g and E tables (or the analytic records) for the in-sphere branch: LDS
     // when they fit.  Separate call sites give each table pointer one address
     // space, so the LDS ones compile to LDS reads.
@@ -236,6 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     } else {
         field_span<AN>(a, a.tab.g, a.tab.e, a.dtypes, j0, j1, W, pbc, p, fs);
     }
+    asm volatile("" ::"v"(w0));
     const FieldRare &st = fs.st;
     const double sx = fs.sx, sy = fs.sy, sz = fs.sz, ex = fs.ex, ey = fs.ey, ez = fs.ez;
     if (!valid) return;
@@ -291,6 +317,14 @@ __device__ __forceinline__ void langevin_moments(double x, double inv_x, double 
     m3 = m1 - 5.0 * m2 * inv_x;
 }
 
+// B splits' loads in flight at a time, combined in split order (a last,
+// partial batch skips the splits past the end): 8 for launches with fewer
+// blocks than CUs, where each batch costs a full memory latency, and 2 for
+// larger ones, where the 8-wide batch's registers would cost occupancy in an
+// HBM-bound pass
+constexpr int kEpiWideGrid = 256;
+
+template <int B>
 __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
 {
     const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
@@ -301,26 +335,29 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     const long long m = gp - (long long)W.a0 * a.nmc;
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
     int cnt = 0;
-    // kEpiBatch splits' loads in flight at a time, combined in split order
-    constexpr int kEpiBatch = 4;
-    int sp = 0;
-    for (; sp + kEpiBatch - 1 < W.splits; sp += kEpiBatch) {
-        float pg[kEpiBatch];
-        int pc[kEpiBatch];
-        double ps[kEpiBatch][3], pe[kEpiBatch][3];
+    for (int sp = 0; sp < W.splits; sp += B) {
+        float pg[B];
+        int pc[B];
+        double ps[B][3], pe[B][3];
 #pragma unroll
-        for (int u = 0; u < kEpiBatch; ++u) {
-            const long long o = W.fpo + (long long)(sp + u) * P + m;
-            pg[u] = a.part_g[o];
-            pc[u] = a.part_cnt[o];
+        for (int u = 0; u < B; ++u) {
+            pg[u] = 1.0f;
+            pc[u] = 0;
+            for (int c = 0; c < 3; ++c) ps[u][c] = pe[u][c] = 0.0;
+            if (sp + u < W.splits) {
+                const long long o = W.fpo + (long long)(sp + u) * P + m;
+                pg[u] = a.part_g[o];
+                pc[u] = a.part_cnt[o];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                ps[u][c] = a.part_s0[c * a.F + o];
-                pe[u][c] = a.part_ein[c * a.F + o];
+                for (int c = 0; c < 3; ++c) {
+                    ps[u][c] = a.part_s0[c * a.F + o];
+                    pe[u][c] = a.part_ein[c * a.F + o];
+                }
             }
         }
 #pragma unroll
-        for (int u = 0; u < kEpiBatch; ++u) {
+        for (int u = 0; u < B; ++u) {
+            if (sp + u >= W.splits) break;
             g *= (double)pg[u];
             cnt += pc[u];
 #pragma unroll
@@ -328,15 +365,6 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
                 s0[c] += ps[u][c];
                 ei[c] += pe[u][c];
             }
-        }
-    }
-    for (; sp < W.splits; ++sp) {
-        const long long o = W.fpo + (long long)sp * P + m;
-        g *= (double)a.part_g[o];
-        cnt += a.part_cnt[o];
-        for (int c = 0; c < 3; ++c) {
-            s0[c] += a.part_s0[c * a.F + o];
-            ei[c] += a.part_ein[c * a.F + o];
         }
     }
     double E0[3], E[3];
@@ -715,7 +743,11 @@ hipError_t launch_field(const FieldArgs &a0, int nblocks, hipStream_t s)
 hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s)
 {
     if (a.Ptot == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_field_epilogue, dim3(blocks_for(a.Ptot)), dim3(kBlock), 0, s, a);
+    const int eb = (int)blocks_for(a.Ptot);
+    if (eb < kEpiWideGrid)
+        hipLaunchKernelGGL(k_field_epilogue<8>, dim3(eb), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_field_epilogue<2>, dim3(eb), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

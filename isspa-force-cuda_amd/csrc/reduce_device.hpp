@@ -19,7 +19,7 @@ constexpr int kSumSlices = 8;
 
 // Chunks whose partial loads one slice keeps in flight at a time; the
 // batch only groups the loads, the adds stay in chunk order.
-constexpr int kSumBatch = 8;
+constexpr int kSumBatch = 4;
 
 // IS-SPA force partials of atom i (COMPS = 6, or 9 in split mode), chunks of
 // slice sl, added in chunk order.

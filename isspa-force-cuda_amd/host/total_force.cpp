@@ -133,7 +133,7 @@ int main(int argc, char *argv[])
         isspa_check(md_engine_set_step(eng, start, stream), "md_engine_set_step");
         // several steps per graph replay when they divide the write interval,
         // so every write still falls on a replay boundary
-        for (int k : {10, 5, 2})
+        for (int k : {50, 20, 10, 5, 2})
             if (configs.deltaWrite % k == 0 && configs.nSteps >= k) {
                 steps_per_graph = k;
                 break;

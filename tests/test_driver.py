@@ -120,17 +120,17 @@ def test_driver_umbrella_cv_in_eager_and_graph_modes(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n_steps,delta_ten", [(23, 10), (15, 20)])
+@pytest.mark.parametrize("n_steps,delta_ten", [(23, 10), (15, 20), (53, 100)])
 def test_driver_graph_steps_per_replay_leave_the_trajectory_unchanged(tmp_path, n_steps,
                                                                        delta_ten):
-    """Graph mode replays 10 steps per graph when they divide the write
-    interval, and the last steps (23 = 2 x 10 + 3, or 15 = 10 + 5) with a
-    one-step graph.  With 15 steps and writes every 20 the one-step graph is
-    captured right after the 10-step replay is launched, with no write (and so
-    no synchronisation) between them: the recapture must wait for that replay
-    before it frees its graph.  The trajectory equals, digit for digit, the
-    run that replays one step per graph (write interval 23: no divisor among
-    10, 5, 2)."""
+    """Graph mode replays the largest of 50, 20, 10, 5, 2 steps per graph that
+    divides the write interval and fits the run, and the last steps
+    (23 = 2 x 10 + 3, 15 = 10 + 5, 53 = 50 + 3) with a one-step graph.  With
+    15 steps and writes every 20 the one-step graph is captured right after
+    the 10-step replay is launched, with no write (and so no synchronisation)
+    between them: the recapture must wait for that replay before it frees its
+    graph.  The trajectory equals, digit for digit, the run that replays one
+    step per graph (write interval 23: no divisor among them)."""
     out = {}
     for name, delta in (("ten", delta_ten), ("one", 23)):
         d = tmp_path / name
