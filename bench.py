@@ -54,8 +54,55 @@ FLOP_FIELD_PAIR = 24      # field pass, per (point, atom), outside rmax
 FLOP_NB_PAIR = 36         # nonbond, per unordered pair
 
 
+TRAFFIC_JSON = "k_force_19998atoms_nmc20_traffic.json"
+COUNTERS_JSON = "isspa_passes_19998atoms_nmc20_valu_counters.json"
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_at(chunks, path):
+    """HBM-side bytes of one k_force launch at `chunks` force chunks, from the
+    committed FETCH_SIZE / WRITE_SIZE passes (copy-calibrated) at several
+    chunk counts.  k_force reads the point records and writes 6 FP64 partials
+    per (chunk, atom): its WRITE_SIZE is linear in the chunk count and its
+    FETCH_SIZE nearly constant, so both are interpolated linearly between the
+    two profiled counts around `chunks`.  Returns (bytes or None outside the
+    profiled range, note, summary file), or None without the file."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tr = json.load(fh)
+    pts = sorted(tr["points"], key=lambda q: q["chunks"])
+    traffic, note = None, (f"outside the profiled chunk counts "
+                           f"{pts[0]['chunks']}-{pts[-1]['chunks']}; this run used {chunks}")
+    for lo, hi in zip(pts, pts[1:]):
+        if lo["chunks"] <= chunks <= hi["chunks"]:
+            t = (chunks - lo["chunks"]) / max(1, hi["chunks"] - lo["chunks"])
+            kib = lambda k: lo[k] + t * (hi[k] - lo[k])
+            traffic = int(round((kib("fetch_kib") * tr["fetch_scale"] +
+                                 kib("write_kib") * tr["write_scale"]) * 1024))
+            note = (f"FETCH_SIZE/WRITE_SIZE interpolated between the profiled {lo['chunks']} and "
+                    f"{hi['chunks']} force chunks (this run {chunks})")
+            break
+    return traffic, note, tr["summary"]
+
+
+def counter_check(path):
+    """Per wave-pair VALU instructions and executed-to-algorithmic FLOP of
+    the force and field passes, from the committed counter pass (or None)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        vc = json.load(fh)
+    out = {k: {"valu_per_wave_pair": vc[k]["valu_per_wave_pair"],
+               "executed_flop_per_pair": vc[k]["executed_flop_per_pair"],
+               "algorithmic_flop_per_pair": vc[k]["algorithmic_flop_per_pair"],
+               "executed_over_algorithmic": vc[k]["executed_over_algorithmic"]}
+           for k in ("k_force", "k_field")}
+    out["source"] = vc["summary"]
+    return out
 
 
 def cpu_model():
@@ -421,44 +468,16 @@ def main():
         out["config"]["autotune"] = {"kind": tune.kind, "value": list(tune.value),
                                      "ms": round(tune.ms, 4), "measure": tune.measure,
                                      "candidates": len(tune.table)}
-    # HBM-side bytes per k_force launch from the committed FETCH_SIZE /
-    # WRITE_SIZE passes (copy-calibrated) at several force chunk counts.
-    # k_force reads the point records and writes 6 FP64 partials per (chunk,
-    # atom): its WRITE_SIZE is linear in the chunk count and its FETCH_SIZE
-    # nearly constant, so both are interpolated linearly between the two
-    # profiled counts around this run's; outside the profiled range: null
-    tr_path = os.path.join(ROOT, "profiles", "k_force_19998atoms_nmc20_traffic.json")
-    if args.workload == 2 and os.path.exists(tr_path):
-        with open(tr_path) as fh:
-            tr = json.load(fh)
-        pts = sorted(tr["points"], key=lambda q: q["chunks"])
-        traffic, note = None, (f"outside the profiled chunk counts "
-                               f"{pts[0]['chunks']}-{pts[-1]['chunks']}; this run used {chunks}")
-        for lo, hi in zip(pts, pts[1:]):
-            if lo["chunks"] <= chunks <= hi["chunks"]:
-                t = (chunks - lo["chunks"]) / max(1, hi["chunks"] - lo["chunks"])
-                kib = lambda k: lo[k] + t * (hi[k] - lo[k])
-                traffic = int(round((kib("fetch_kib") * tr["fetch_scale"] +
-                                     kib("write_kib") * tr["write_scale"]) * 1024))
-                note = (f"FETCH_SIZE/WRITE_SIZE interpolated between the profiled {lo['chunks']} and "
-                        f"{hi['chunks']} force chunks (this run {chunks})")
-                break
-        out["roofline"]["traffic"] = traffic
-        out["roofline"]["traffic_note"] = note
-        out["roofline"]["traffic_source"] = tr["summary"]
-    # the VALU-instruction counter pass of the same kernels (committed)
-    vc_path = os.path.join(ROOT, "profiles", "isspa_passes_19998atoms_nmc20_valu_counters.json")
-    if args.workload == 2 and os.path.exists(vc_path):
-        with open(vc_path) as fh:
-            vc = json.load(fh)
-        out["roofline"]["counter_check"] = {
-            k: {"valu_per_wave_pair": vc[k]["valu_per_wave_pair"],
-                "executed_flop_per_pair": vc[k]["executed_flop_per_pair"],
-                "algorithmic_flop_per_pair": vc[k]["algorithmic_flop_per_pair"],
-                "executed_over_algorithmic": vc[k]["executed_over_algorithmic"]}
-            for k in ("k_force", "k_field")}
-        out["roofline"]["counter_check"]["source"] = vc["summary"]
-
+    # HBM-side bytes per k_force launch and the VALU-instruction counter pass
+    # of the same kernels, from the committed profiles
+    if args.workload == 2:
+        tr = traffic_at(chunks, os.path.join(ROOT, "profiles", TRAFFIC_JSON))
+        if tr is not None:
+            out["roofline"]["traffic"], out["roofline"]["traffic_note"], \
+                out["roofline"]["traffic_source"] = tr
+        cc = counter_check(os.path.join(ROOT, "profiles", COUNTERS_JSON))
+        if cc is not None:
+            out["roofline"]["counter_check"] = cc
     g.close()
     if not args.no_batch:
         out["batched"] = batched_bench(args.batch_steps, args.plain_rounds, args.warmup,
