@@ -172,7 +172,7 @@ static int isspa_tile(isspa_model *m)
     }
     if (C > m->cap_force) {
         dfree(m->part_f);
-        CHK(dalloc(&m->part_f, 9 * C));
+        CHK(dalloc(&m->part_f, (size_t)force_part_stride(true) * C));
         m->cap_force = C;
     }
     return ISSPA_OK;

@@ -105,7 +105,12 @@ __global__ __launch_bounds__(kB) void k_diff_reduce(DiffReduceArgs a)
     double s[6] = {0, 0, 0, 0, 0, 0};
     for (int c = 0; c < W.chunks; ++c) {
         const long long o = W.ppo + (long long)c * W.n + il;
-        for (int k = 0; k < 6; ++k) s[k] += a.part[k * a.C + o];
+        const double2 *e = reinterpret_cast<const double2 *>(a.part + o * force_part_stride(false));
+        for (int h = 0; h < 3; ++h) {
+            const double2 d = e[h];
+            s[2 * h] += d.x;
+            s[2 * h + 1] += d.y;
+        }
     }
     const double vt = a.vtot[a.itype[i]];
     const double qv = -a.q[i] * vt;

@@ -642,17 +642,13 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     asm volatile("" ::"v"(w0), "v"(w1));
     if (!valid) return;
     const long long o = W.ppo + (long long)chunk * W.n + il;
-    const long long C = a.C;
-    a.part[o] = acc.ex;
-    a.part[C + o] = acc.ey;
-    a.part[2 * C + o] = acc.ez;
-    a.part[3 * C + o] = acc.lx;
-    a.part[4 * C + o] = acc.ly;
-    a.part[5 * C + o] = acc.lz;
+    double2 *e = reinterpret_cast<double2 *>(a.part + o * force_part_stride(SPLIT));
+    e[0] = make_double2(acc.ex, acc.ey);
+    e[1] = make_double2(acc.ez, acc.lx);
+    e[2] = make_double2(acc.ly, acc.lz);
     if (SPLIT) {
-        a.part[6 * C + o] = acc.cx;
-        a.part[7 * C + o] = acc.cy;
-        a.part[8 * C + o] = acc.cz;
+        e[3] = make_double2(acc.cx, acc.cy);
+        e[4] = make_double2(acc.cz, 0.0);
     }
 }
 

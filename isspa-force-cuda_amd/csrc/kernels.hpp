@@ -126,7 +126,7 @@ struct ForceArgs {
     const DiffType *dtypes;  // non-null: analytic F instead of the table
     int nmc;
     const int *pbc_flag;
-    double *part;         // [comps][C] (S_e, S_lj) or (pol, lj, cdd)
+    double *part;         // [C][stride]: (S_e, S_lj), or (pol, lj, cdd, pad) split
     long long C;
     int xcd_runs;         // 1: each XCD takes a contiguous run of the table (equal work items)
 };
@@ -140,6 +140,12 @@ __device__ __forceinline__ int xcd_run_entry(int b, int nb)
     const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
     return x * q + (x < r ? x : r) + k;
 }
+
+// The force pass's chunk partials are stored entry by entry, entry
+// (chunk, atom) holding its components together: 6 doubles (S_e, S_lj), or
+// 10 in split mode (pol, lj, cdd and a pad, so every entry stays 16-byte
+// aligned), read and written as double2.
+__host__ __device__ constexpr int force_part_stride(bool split) { return split ? 10 : 6; }
 
 struct ReduceArgs {
     const WinDesc *wins;

@@ -19,10 +19,25 @@ constexpr int kSumSlices = 8;
 
 // Chunks whose partial loads one slice keeps in flight at a time; the
 // batch only groups the loads, the adds stay in chunk order.
-constexpr int kSumBatch = 4;
+constexpr int kSumBatch = 6;
 
 // IS-SPA force partials of atom i (COMPS = 6, or 9 in split mode), chunks of
 // slice sl, added in chunk order.
+// the COMPS components of chunk-partial entry o (stored as double2 pairs)
+template <int COMPS>
+__device__ __forceinline__ void load_force_entry(const double *part, long long o,
+                                                 double (&v)[COMPS])
+{
+    constexpr int kStride = force_part_stride(COMPS == 9);
+    const double2 *e = reinterpret_cast<const double2 *>(part + o * kStride);
+#pragma unroll
+    for (int h = 0; h < (COMPS + 1) / 2; ++h) {
+        const double2 d = e[h];
+        v[2 * h] = d.x;
+        if (2 * h + 1 < COMPS) v[2 * h + 1] = d.y;
+    }
+}
+
 template <int COMPS>
 __device__ inline void isspa_slice_sum(const ReduceArgs &a, int i, int sl, double (&s)[COMPS])
 {
@@ -32,20 +47,18 @@ __device__ inline void isspa_slice_sum(const ReduceArgs &a, int i, int sl, doubl
     for (; c + (kSumBatch - 1) * kSumSlices < W.chunks; c += kSumBatch * kSumSlices) {
         double v[kSumBatch][COMPS];
 #pragma unroll
-        for (int u = 0; u < kSumBatch; ++u) {
-            const long long o = base + (long long)(c + u * kSumSlices) * W.n;
-#pragma unroll
-            for (int k = 0; k < COMPS; ++k) v[u][k] = a.part[k * a.C + o];
-        }
+        for (int u = 0; u < kSumBatch; ++u)
+            load_force_entry<COMPS>(a.part, base + (long long)(c + u * kSumSlices) * W.n, v[u]);
 #pragma unroll
         for (int u = 0; u < kSumBatch; ++u)
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
     }
     for (; c < W.chunks; c += kSumSlices) {
-        const long long o = base + (long long)c * W.n;
+        double v[COMPS];
+        load_force_entry<COMPS>(a.part, base + (long long)c * W.n, v);
 #pragma unroll
-        for (int k = 0; k < COMPS; ++k) s[k] += a.part[k * a.C + o];
+        for (int k = 0; k < COMPS; ++k) s[k] += v[k];
     }
 }
 
