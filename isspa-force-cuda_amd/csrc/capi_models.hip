@@ -117,6 +117,7 @@ static int isspa_tile(isspa_model *m)
             : (double)std::min(kMaxAutoWaves,
                                std::max(kMinAutoWaves, (long)(m->P / kPointsPerWave)));
     m->wins_h.assign(L.W, WinDesc{});
+    m->max_win_atoms = m->max_chunk_pts = 0;
     long long F = 0, C = 0;
     std::vector<std::pair<long long, int4>> fb, pb;
     for (int w = 0; w < L.W; ++w) {
@@ -131,6 +132,8 @@ static int isspa_tile(isspa_model *m)
         c = (int)std::max(1LL, std::min((long long)c, P));
         d.ppc = (int)((P + c - 1) / c);
         d.chunks = (int)((P + d.ppc - 1) / d.ppc);
+        m->max_win_atoms = std::max(m->max_win_atoms, (int)n);
+        m->max_chunk_pts = std::max(m->max_chunk_pts, d.ppc);
         d.fpo = F;
         d.ppo = C;
         F += (long long)d.splits * P;
@@ -1181,8 +1184,14 @@ static int engine_forces(md_engine *e, hipStream_t s, bool integrate)
             fa.nb_blocks = e->nb->nblk;
         }
         HIPCHK(launch_field(fa, m->nfblk, s));
-        HIPCHK(launch_epilogue(epilogue_args(m, false), s));
-        HIPCHK(launch_force(force_args(m, x4), m->npblk, false, s));
+        const ForceArgs fo = force_args(m, x4);
+        if (!fo.dtypes && force_epi_fits(fo.tab, m->max_win_atoms, m->max_chunk_pts, m->npblk)) {
+            // small windows: the epilogue inside the force launch
+            HIPCHK(launch_force_epi(fo, epilogue_args(m, false), m->npblk, s));
+        } else {
+            HIPCHK(launch_epilogue(epilogue_args(m, false), s));
+            HIPCHK(launch_force(fo, m->npblk, false, s));
+        }
         a.isspa = reduce_args(m, x4, nullptr, nullptr, nullptr, nullptr);
     }
     if (fork)

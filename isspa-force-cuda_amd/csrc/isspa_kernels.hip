@@ -317,22 +317,27 @@ __device__ __forceinline__ void langevin_moments(double x, double inv_x, double 
     m3 = m1 - 5.0 * m2 * inv_x;
 }
 
-// B splits' loads in flight at a time, combined in split order (a last,
-// partial batch skips the splits past the end): 8 for launches with fewer
-// blocks than CUs, where each batch costs a full memory latency, and 2 for
-// larger ones, where the 8-wide batch's registers would cost occupancy in an
-// HBM-bound pass
-constexpr int kEpiWideGrid = 256;
+// One point's epilogue: its split partials summed in split order (B
+// splits' loads in flight at a time), then the FP64 Langevin moments, the
+// 16-float force record {p, Ehat, V, D, A2, 3 A2, A3, w} and the aux entry
+// {cp E0, A1}.  The epilogue kernel writes them to memory; the fused force
+// kernel of small systems keeps them in LDS.  Both run this same code, so
+// the records are the same bit for bit.
+struct PointRecord {
+    float4 r[4];
+    float4 aux;
+    double g;
+    int cnt;
+    double E[3], E0[3];
+};
 
 template <int B>
-__global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
+__device__ __forceinline__ void epilogue_point(const EpilogueArgs &a, const WinDesc &W,
+                                               long long gp, PointRecord &out)
 {
-    const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (gp >= a.Ptot) return;
-    // one window: no per-point window lookup
-    const WinDesc W = a.nwin == 1 ? a.wins[0] : a.wins[a.atom_win[gp / a.nmc]];
     const int P = W.n * a.nmc;
     const long long m = gp - (long long)W.a0 * a.nmc;
+    const float4 p = a.pts[gp];  // requested with the partials, used at the end
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
     int cnt = 0;
     for (int sp = 0; sp < W.splits; sp += B) {
@@ -387,23 +392,47 @@ __global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
     const double A3 = (double)a.o0 * m3 * w;
     double c3[3];  // cp E0 / 3
     for (int c = 0; c < 3; ++c) c3[c] = cp * E0[c] * (1.0 / 3.0);
-    const float4 p = a.pts[gp];
+    out.r[0] = make_float4(p.x, p.y, p.z, (float)eh[0]);
+    out.r[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + c3[0]),
+                           (float)(A1 * eh[1] + c3[1]));
+    out.r[2] = make_float4((float)(A1 * eh[2] + c3[2]),
+                           (float)(3.0 * A1 * eh[0] + cp * E0[0]),
+                           (float)(3.0 * A1 * eh[1] + cp * E0[1]),
+                           (float)(3.0 * A1 * eh[2] + cp * E0[2]));
+    out.r[3] = make_float4((float)A2, (float)(3.0 * A2), (float)A3, (float)w);
+    out.aux = make_float4((float)(cp * E0[0]), (float)(cp * E0[1]), (float)(cp * E0[2]), (float)A1);
+    out.g = g;
+    out.cnt = cnt;
+    for (int c = 0; c < 3; ++c) {
+        out.E[c] = E[c];
+        out.E0[c] = E0[c];
+    }
+}
+
+// B splits' loads in flight at a time: 8 for launches with fewer blocks than
+// CUs, where each batch costs a full memory latency, and 2 for larger ones,
+// where the 8-wide batch's registers would cost occupancy in an HBM-bound
+// pass
+constexpr int kEpiWideGrid = 256;
+
+template <int B>
+__global__ __launch_bounds__(kBlock) void k_field_epilogue(EpilogueArgs a)
+{
+    const long long gp = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (gp >= a.Ptot) return;
+    // one window: no per-point window lookup
+    const WinDesc W = a.nwin == 1 ? a.wins[0] : a.wins[a.atom_win[gp / a.nmc]];
+    PointRecord pr;
+    epilogue_point<B>(a, W, gp, pr);
     float4 *r4 = reinterpret_cast<float4 *>(a.rec + gp * kRecFloats);
-    r4[0] = make_float4(p.x, p.y, p.z, (float)eh[0]);
-    r4[1] = make_float4((float)eh[1], (float)eh[2], (float)(A1 * eh[0] + c3[0]),
-                        (float)(A1 * eh[1] + c3[1]));
-    r4[2] = make_float4((float)(A1 * eh[2] + c3[2]),
-                        (float)(3.0 * A1 * eh[0] + cp * E0[0]),
-                        (float)(3.0 * A1 * eh[1] + cp * E0[1]),
-                        (float)(3.0 * A1 * eh[2] + cp * E0[2]));
-    r4[3] = make_float4((float)A2, (float)(3.0 * A2), (float)A3, (float)w);
-    a.aux[gp] = make_float4((float)(cp * E0[0]), (float)(cp * E0[1]), (float)(cp * E0[2]), (float)A1);
-    if (a.g_out) a.g_out[gp] = g;
-    if (a.cnt_out) a.cnt_out[gp] = cnt;
+    for (int k = 0; k < 4; ++k) r4[k] = pr.r[k];
+    a.aux[gp] = pr.aux;
+    if (a.g_out) a.g_out[gp] = pr.g;
+    if (a.cnt_out) a.cnt_out[gp] = pr.cnt;
     if (a.E_out)
-        for (int c = 0; c < 3; ++c) a.E_out[3 * gp + c] = E[c];
+        for (int c = 0; c < 3; ++c) a.E_out[3 * gp + c] = pr.E[c];
     if (a.E0_out)
-        for (int c = 0; c < 3; ++c) a.E0_out[3 * gp + c] = E0[c];
+        for (int c = 0; c < 3; ++c) a.E0_out[3 * gp + c] = pr.E0[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -559,12 +588,13 @@ __device__ __forceinline__ void force_chunk(const ForceArgs &a, const float4 *__
     for (int mt = m0; mt < m1; mt += kForceTile) {
         const int me = min(m1, mt + kForceTile);
         TileSums ts{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        // tile-local point indices from the tile's first record
         if (pbc)
-            force_tile<true, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
-                                        fp, ts, acc);
+            force_tile<true, SPLIT, AN>(a, rec + 4 * mt, aux + mt, 0, me - mt, W.L, W.h, xi.x,
+                                        xi.y, xi.z, R2, Ftab, fp, ts, acc);
         else
-            force_tile<false, SPLIT, AN>(a, rec, aux, mt, me, W.L, W.h, xi.x, xi.y, xi.z, R2, Ftab,
-                                         fp, ts, acc);
+            force_tile<false, SPLIT, AN>(a, rec + 4 * mt, aux + mt, 0, me - mt, W.L, W.h, xi.x,
+                                         xi.y, xi.z, R2, Ftab, fp, ts, acc);
         acc.ex += (double)ts.sx;
         acc.ey += (double)ts.sy;
         acc.ez += (double)ts.sz;
@@ -650,6 +680,64 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
         e[3] = make_double2(acc.cx, acc.cy);
         e[4] = make_double2(acc.cz, 0.0);
     }
+}
+
+// Small systems (every window within one receiver block, at most
+// kFuseMaxPts points a chunk, at most kFuseMaxBlocks blocks): the epilogue
+// and the force pass in one launch.  The block runs the epilogue of its
+// chunk's points (no other block needs them: the chunk has one receiver
+// block) into LDS, then the force loop reads the records from there.
+// Records and force sums are those of the two launches bit for bit; the
+// records are not written to memory.  The epilogue's FP64 registers allow
+// two blocks per CU, so the launch is fused only when its blocks fit one
+// round of them: beyond that the separate launches are faster.
+constexpr int kFuseMaxPts = 256;
+constexpr int kFuseMaxBlocks = 512;
+
+__host__ __device__ inline int fused_lds_floats(const TableView &t)
+{
+    return ((force_tab_floats(t) + 3) & ~3) + kFuseMaxPts * (kRecFloats + 4);
+}
+
+__global__ __launch_bounds__(kBlock) void k_force_epi(ForceArgs a, EpilogueArgs ea)
+{
+    const int4 bt = a.blocks[a.xcd_runs ? xcd_run_entry(blockIdx.x, gridDim.x) : blockIdx.x];
+    const WinDesc W = a.wins[bt.x];
+    const int il = bt.y * kBlock + threadIdx.x;
+    const int chunk = bt.z;
+    const long long pbase = (long long)W.a0 * a.nmc;
+    const int m0 = chunk * W.ppc;
+    const int np = min(W.n * a.nmc, m0 + W.ppc) - m0;
+    const bool valid = il < W.n;
+    const int i = W.a0 + (valid ? il : 0);
+    const float4 xi = a.xyzq[i];
+    const int ti = a.itype[i];
+    const float R2 = valid ? a.rmax2[ti] : -1.0f;
+    extern __shared__ float lds_tab[];
+    const int nt = force_tab_floats(a.tab);
+    float4 *lrec = reinterpret_cast<float4 *>(lds_tab + ((nt + 3) & ~3));
+    float4 *laux = lrec + 4 * kFuseMaxPts;
+    for (int t = threadIdx.x; t < np; t += kBlock) {
+        PointRecord pr;
+        epilogue_point<4>(ea, W, pbase + m0 + t, pr);
+        for (int k = 0; k < 4; ++k) lrec[4 * t + k] = pr.r[k];
+        laux[t] = pr.aux;
+    }
+    stage_lds(lds_tab, a.tab.f, nt);
+    __syncthreads();
+    const bool wave_idle = bt.y * kBlock + (int)(threadIdx.x & ~(kWave - 1)) >= W.n;
+    if (wave_idle) return;
+    const bool pbc = a.pbc_flag[bt.x] != 0;
+    const DiffFParams fp{0.0f, 0.0f, 0.0f, 0.0f};
+    ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    force_chunk<false, false>(a, lrec, laux, 0, np, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f,
+                              fp, acc);
+    if (!valid) return;
+    const long long o = W.ppo + (long long)chunk * W.n + il;
+    double2 *e = reinterpret_cast<double2 *>(a.part + o * force_part_stride(false));
+    e[0] = make_double2(acc.ex, acc.ey);
+    e[1] = make_double2(acc.ez, acc.lx);
+    e[2] = make_double2(acc.ly, acc.lz);
 }
 
 // F_i = -q_i vtot_i S_e - vtot_i S_lj over the chunk partials, summed in the
@@ -744,6 +832,20 @@ hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s)
         hipLaunchKernelGGL(k_field_epilogue<8>, dim3(eb), dim3(kBlock), 0, s, a);
     else
         hipLaunchKernelGGL(k_field_epilogue<2>, dim3(eb), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+bool force_epi_fits(const TableView &t, int max_window_atoms, int max_chunk_points, int nblocks)
+{
+    return max_window_atoms <= kBlock && max_chunk_points <= kFuseMaxPts &&
+           nblocks <= kFuseMaxBlocks && force_tab_floats(t) <= kTabLdsFloats;
+}
+
+hipError_t launch_force_epi(const ForceArgs &a, const EpilogueArgs &ea, int nblocks, hipStream_t s)
+{
+    if (nblocks == 0) return hipSuccess;
+    const size_t lds = (size_t)fused_lds_floats(a.tab) * sizeof(float);
+    hipLaunchKernelGGL(k_force_epi, dim3(nblocks), dim3(kBlock), lds, s, a, ea);
     return hipGetLastError();
 }
 

@@ -51,6 +51,7 @@ struct isspa_model {
     WinDesc *wins = nullptr;
     int4 *fblk = nullptr, *pblk = nullptr;
     int nfblk = 0, npblk = 0;
+    int max_win_atoms = 0, max_chunk_pts = 0;  // over the windows, for the fused force launch
     float *part_g = nullptr;
     int *part_cnt = nullptr;
     double *part_s0 = nullptr, *part_ein = nullptr;

@@ -156,6 +156,35 @@ def test_standalone_calls_equal_the_engine_bitwise(name, adi2, cfg1):
     g.close()
 
 
+@pytest.mark.parametrize("name,tiling", [("adi2", (5, 200)), ("cfg0", (2, 10)),
+                                         ("cfg0_auto", None)])
+def test_engine_fused_force_launch_equals_standalone_calls_bitwise(name, tiling, adi2, cfg0):
+    """Systems whose windows fit one receiver block (and whose chunks hold at
+    most 256 points, 512 blocks in all) run the epilogue inside the force
+    launch, records in LDS.  Their engine forces must equal zero +
+    bonded_force + isspa_force (epilogue and force pass as two launches,
+    records in memory) + nonbond_force bit for bit.  ADI2 at nMC = 1000 with
+    200 chunks has 220 points a chunk; the 22-atom chain at nMC = 20 has 44
+    with 10 chunks, 16 with the automatic tiling."""
+    s = {"adi2": adi2, "cfg0": cfg0, "cfg0_auto": cfg0}[name]
+    g = _gpu(s, us=False)
+    if tiling is not None:
+        g.set_tiling(*tiling)
+    print(f"case fused force launch {name}: N={s.n_atoms} nMC={s.n_mc} tiling {g.tiling()}",
+          flush=True)
+    assert s.n_atoms <= 256
+    g.set_step(2)
+    got = g.forces().clone()
+    g.zero_forces()
+    g.bonded_force()
+    g.isspa_force(step=2)
+    g.nonbond_force()
+    torch.cuda.synchronize()
+    assert torch.equal(got, g.f)
+    assert torch.all(torch.isfinite(got))
+    g.close()
+
+
 @pytest.mark.parametrize("mode", ["eager", "graph"])
 def test_engine_step_counter_advances_once_per_step(cfg1, mode):
     """The engine keeps its step on the device and the leap-frog launch moves
