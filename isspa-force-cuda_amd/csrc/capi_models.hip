@@ -118,6 +118,7 @@ static int isspa_tile(isspa_model *m)
                                std::max(kMinAutoWaves, (long)(m->P / kPointsPerWave)));
     m->wins_h.assign(L.W, WinDesc{});
     m->max_win_atoms = m->max_chunk_pts = 0;
+    m->min_win_atoms = 1 << 30;
     long long F = 0, C = 0;
     std::vector<std::pair<long long, int4>> fb, pb;
     for (int w = 0; w < L.W; ++w) {
@@ -128,11 +129,15 @@ static int isspa_tile(isspa_model *m)
         s = (int)std::max(1LL, std::min((long long)s, n));
         d.aps = (int)((n + s - 1) / s);
         d.splits = (int)((n + d.aps - 1) / d.aps);
-        int c = m->user_chunks > 0 ? m->user_chunks : auto_pieces(n, P, kMinChunkPoints, tgt);
+        // a chunk keeps kMinChunkPoints points for each receiver replica
+        int c = m->user_chunks > 0
+                    ? m->user_chunks
+                    : auto_pieces(n, P, kMinChunkPoints * force_replicas((int)n), tgt);
         c = (int)std::max(1LL, std::min((long long)c, P));
         d.ppc = (int)((P + c - 1) / c);
         d.chunks = (int)((P + d.ppc - 1) / d.ppc);
         m->max_win_atoms = std::max(m->max_win_atoms, (int)n);
+        m->min_win_atoms = std::min(m->min_win_atoms, (int)n);
         m->max_chunk_pts = std::max(m->max_chunk_pts, d.ppc);
         d.fpo = F;
         d.ppo = C;
@@ -351,6 +356,7 @@ ForceArgs force_args(isspa_model *m, const float4 *x)
     // one system: every block has the same work, so each XCD may take a
     // contiguous run of chunks (batches keep their longest-first order)
     fa.xcd_runs = m->lay.W == 1;
+    fa.replicas = force_replicas(m->min_win_atoms);
     return fa;
 }
 

@@ -604,18 +604,57 @@ __device__ __forceinline__ void force_chunk(const ForceArgs &a, const float4 *__
     }
 }
 
-template <bool SPLIT, bool AN>
+// Replicas 1..r-1 of the block's receivers leave their sums in LDS; replica
+// 0 adds them in replica order.  Every wave of the block reaches the barrier
+// (with r > 1 every wave holds receivers).  Returns false on the waves whose
+// sums were handed over.
+template <bool SPLIT>
+__device__ __forceinline__ bool replica_sum(double *red, int r, int q, int slot, int nslots,
+                                            ForceAcc &acc)
+{
+    constexpr int nc = SPLIT ? 9 : 6;
+    if (q > 0) {
+        double *d = red + ((long long)(q - 1) * nslots + slot) * nc;
+        d[0] = acc.ex; d[1] = acc.ey; d[2] = acc.ez;
+        d[3] = acc.lx; d[4] = acc.ly; d[5] = acc.lz;
+        if (SPLIT) { d[6] = acc.cx; d[7] = acc.cy; d[8] = acc.cz; }
+    }
+    __syncthreads();
+    if (q > 0) return false;
+    for (int k = 1; k < r; ++k) {
+        const double *d = red + ((long long)(k - 1) * nslots + slot) * nc;
+        acc.ex += d[0]; acc.ey += d[1]; acc.ez += d[2];
+        acc.lx += d[3]; acc.ly += d[4]; acc.lz += d[5];
+        if (SPLIT) { acc.cx += d[6]; acc.cy += d[7]; acc.cz += d[8]; }
+    }
+    return true;
+}
+
+// REP: some window of the launch has receiver replicas (the replica code is
+// compiled only into that instantiation, so the large-system kernel keeps its
+// registers)
+template <bool SPLIT, bool AN, bool REP>
 __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 {
     const int4 bt = a.blocks[a.xcd_runs ? xcd_run_entry(blockIdx.x, gridDim.x) : blockIdx.x];
     const WinDesc W = a.wins[bt.x];
-    const int il = bt.y * kBlock + threadIdx.x;  // receiver within the window
+    // receiver replicas: wave wv serves receivers (wv % nrw) * 64 + lane of
+    // the block on the q-th share of the chunk's points
+    const int r = REP ? force_replicas(W.n) : 1, nrw = 4 / r;
+    // the wave index read from the first lane, so the compiler knows it is
+    // wave-uniform and the replica's point range stays on scalar loads
+    const int wv = REP ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave) : 0;
+    const int q = REP ? wv / nrw : 0;
+    const int slot = REP ? (wv % nrw) * kWave + (int)threadIdx.x % kWave : (int)threadIdx.x;
+    const int il = bt.y * kBlock + slot;  // receiver within the window
     const int chunk = bt.z;
     const long long pbase = (long long)W.a0 * a.nmc;
     const float4 *__restrict__ rec = a.rec4 + 4 * pbase;  // this window's records
     const float4 *__restrict__ aux = a.aux4 + pbase;
     const int m0 = chunk * W.ppc;                        // window-local point range
     const int m1 = min(W.n * a.nmc, m0 + W.ppc);
+    const int sub = (m1 - m0 + r - 1) / r;               // this replica's share
+    const int ms = min(m1, m0 + q * sub), me = min(m1, ms + sub);
     const bool valid = il < W.n;
     const int i = W.a0 + (valid ? il : 0);
     const float4 xi = a.xyzq[i];
@@ -657,19 +696,25 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
     // the LDS one compiles to LDS reads
     // a wave with no receiver of the window (small windows) stops after the
     // block's one barrier instead of running the point loop for nothing
-    const bool wave_idle = bt.y * kBlock + (int)(threadIdx.x & ~(kWave - 1)) >= W.n;
+    // (with replicas every wave holds receivers, so none leaves before the
+    // replica barrier)
+    const bool wave_idle = bt.y * kBlock + (slot & ~(kWave - 1)) >= W.n;
     if (!AN && force_tab_floats(a.tab) <= kTabLdsFloats) {
         stage_lds(lds_tab, a.tab.f, force_tab_floats(a.tab));
         __syncthreads();
         if (wave_idle) return;
-        force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f, fp,
+        force_chunk<SPLIT, AN>(a, rec, aux, ms, me, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f, fp,
                                acc);
     } else {
         if (wave_idle) return;
-        force_chunk<SPLIT, AN>(a, rec, aux, m0, m1, W, pbc, xi, R2,
+        force_chunk<SPLIT, AN>(a, rec, aux, ms, me, W, pbc, xi, R2,
                                AN ? nullptr : a.tab.f + ti * a.tab.ld_f, fp, acc);
     }
     asm volatile("" ::"v"(w0), "v"(w1));
+    if (REP && r > 1 &&
+        !replica_sum<SPLIT>(reinterpret_cast<double *>(lds_tab + a.red_off), r, q, slot,
+                            nrw * kWave, acc))
+        return;
     if (!valid) return;
     const long long o = W.ppo + (long long)chunk * W.n + il;
     double2 *e = reinterpret_cast<double2 *>(a.part + o * force_part_stride(SPLIT));
@@ -703,11 +748,18 @@ __global__ __launch_bounds__(kBlock) void k_force_epi(ForceArgs a, EpilogueArgs 
 {
     const int4 bt = a.blocks[a.xcd_runs ? xcd_run_entry(blockIdx.x, gridDim.x) : blockIdx.x];
     const WinDesc W = a.wins[bt.x];
-    const int il = bt.y * kBlock + threadIdx.x;
+    const int r = force_replicas(W.n), nrw = 4 / r;
+    // the wave index read from the first lane, so the compiler knows it is
+    // wave-uniform and the replica's point range stays on scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave), q = wv / nrw;
+    const int slot = (wv % nrw) * kWave + (int)threadIdx.x % kWave;
+    const int il = bt.y * kBlock + slot;
     const int chunk = bt.z;
     const long long pbase = (long long)W.a0 * a.nmc;
     const int m0 = chunk * W.ppc;
     const int np = min(W.n * a.nmc, m0 + W.ppc) - m0;
+    const int sub = (np + r - 1) / r;
+    const int ms = min(np, q * sub), me = min(np, ms + sub);  // chunk-local
     const bool valid = il < W.n;
     const int i = W.a0 + (valid ? il : 0);
     const float4 xi = a.xyzq[i];
@@ -725,13 +777,16 @@ __global__ __launch_bounds__(kBlock) void k_force_epi(ForceArgs a, EpilogueArgs 
     }
     stage_lds(lds_tab, a.tab.f, nt);
     __syncthreads();
-    const bool wave_idle = bt.y * kBlock + (int)(threadIdx.x & ~(kWave - 1)) >= W.n;
+    const bool wave_idle = bt.y * kBlock + (wv % nrw) * kWave >= W.n;
     if (wave_idle) return;
     const bool pbc = a.pbc_flag[bt.x] != 0;
     const DiffFParams fp{0.0f, 0.0f, 0.0f, 0.0f};
     ForceAcc acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    force_chunk<false, false>(a, lrec, laux, 0, np, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f,
+    force_chunk<false, false>(a, lrec, laux, ms, me, W, pbc, xi, R2, lds_tab + ti * a.tab.ld_f,
                               fp, acc);
+    if (r > 1 && !replica_sum<false>(reinterpret_cast<double *>(lds_tab + a.red_off), r, q, slot,
+                                     nrw * kWave, acc))
+        return;
     if (!valid) return;
     const long long o = W.ppo + (long long)chunk * W.n + il;
     double2 *e = reinterpret_cast<double2 *>(a.part + o * force_part_stride(false));
@@ -844,26 +899,35 @@ bool force_epi_fits(const TableView &t, int max_window_atoms, int max_chunk_poin
 hipError_t launch_force_epi(const ForceArgs &a, const EpilogueArgs &ea, int nblocks, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    const size_t lds = (size_t)fused_lds_floats(a.tab) * sizeof(float);
-    hipLaunchKernelGGL(k_force_epi, dim3(nblocks), dim3(kBlock), lds, s, a, ea);
+    ForceArgs b = a;
+    b.red_off = fused_lds_floats(a.tab);  // a multiple of 4 floats
+    const int red = a.replicas > 1 ? 2 * force_replica_doubles(false) : 0;
+    const size_t lds = (size_t)(b.red_off + red) * sizeof(float);
+    hipLaunchKernelGGL(k_force_epi, dim3(nblocks), dim3(kBlock), lds, s, b, ea);
     return hipGetLastError();
 }
 
 hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s)
 {
     if (nblocks == 0) return hipSuccess;
-    if (a.dtypes) {
-        if (split)
-            hipLaunchKernelGGL((k_force<true, true>), dim3(nblocks), dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_force<false, true>), dim3(nblocks), dim3(kBlock), 0, s, a);
-    } else {
-        const int nt = force_tab_floats(a.tab);
-        const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
-        if (split)
-            hipLaunchKernelGGL((k_force<true, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
-        else
-            hipLaunchKernelGGL((k_force<false, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
+    // dynamic LDS: the F table when it fits (table model), then the
+    // replica-sum area when some window has receiver replicas
+    ForceArgs b = a;
+    const int nt = force_tab_floats(a.tab);
+    b.red_off = (!a.dtypes && nt <= kTabLdsFloats) ? (nt + 3) & ~3 : 0;
+    const int red = a.replicas > 1 ? 2 * force_replica_doubles(split) : 0;
+    const size_t lds = (size_t)(b.red_off + red) * sizeof(float);
+    const dim3 g(nblocks), t(kBlock);
+    const int v = (a.dtypes ? 4 : 0) + (split ? 2 : 0) + (a.replicas > 1 ? 1 : 0);
+    switch (v) {
+    case 0: hipLaunchKernelGGL((k_force<false, false, false>), g, t, lds, s, b); break;
+    case 1: hipLaunchKernelGGL((k_force<false, false, true>), g, t, lds, s, b); break;
+    case 2: hipLaunchKernelGGL((k_force<true, false, false>), g, t, lds, s, b); break;
+    case 3: hipLaunchKernelGGL((k_force<true, false, true>), g, t, lds, s, b); break;
+    case 4: hipLaunchKernelGGL((k_force<false, true, false>), g, t, lds, s, b); break;
+    case 5: hipLaunchKernelGGL((k_force<false, true, true>), g, t, lds, s, b); break;
+    case 6: hipLaunchKernelGGL((k_force<true, true, false>), g, t, lds, s, b); break;
+    default: hipLaunchKernelGGL((k_force<true, true, true>), g, t, lds, s, b); break;
     }
     return hipGetLastError();
 }

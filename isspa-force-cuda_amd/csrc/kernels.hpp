@@ -129,7 +129,22 @@ struct ForceArgs {
     double *part;         // [C][stride]: (S_e, S_lj), or (pol, lj, cdd, pad) split
     long long C;
     int xcd_runs;         // 1: each XCD takes a contiguous run of the table (equal work items)
+    int replicas;         // most receiver replicas of any window (force_replicas of the smallest)
+    int red_off;          // float offset of the replica-sum area in the dynamic LDS (set at launch)
 };
+
+// Windows of at most 128 atoms fill only one or two of a 256-thread force
+// block's four waves with receivers.  The other waves then take a share of
+// the chunk's points for the same receivers (4 replicas of the receivers at
+// <= 64 atoms, 2 at <= 128), and the block adds the replicas' sums in replica
+// order before it writes the chunk's partial: the same parallelism with a
+// quarter (or half) of the chunks, so a quarter of the partials to reduce.
+__host__ __device__ constexpr int force_replicas(int n) { return n <= 64 ? 4 : (n <= 128 ? 2 : 1); }
+// doubles of the replica-sum area: replicas 1..r-1 of 64 * (4 / r) receivers
+__host__ __device__ constexpr int force_replica_doubles(bool split)
+{
+    return 3 * 64 * (split ? 9 : 6);
+}
 
 // Workgroup b of a launch goes to XCD b % 8 (round-robin).  With xcd_runs the
 // table entry of workgroup b is the (b / 8)-th of XCD b % 8's contiguous run,

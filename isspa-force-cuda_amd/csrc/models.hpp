@@ -52,6 +52,7 @@ struct isspa_model {
     int4 *fblk = nullptr, *pblk = nullptr;
     int nfblk = 0, npblk = 0;
     int max_win_atoms = 0, max_chunk_pts = 0;  // over the windows, for the fused force launch
+    int min_win_atoms = 0;                     // the smallest window: receiver replicas
     float *part_g = nullptr;
     int *part_cnt = nullptr;
     double *part_s0 = nullptr, *part_ein = nullptr;
