@@ -733,11 +733,12 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 // chunk's points (no other block needs them: the chunk has one receiver
 // block) into LDS, then the force loop reads the records from there.
 // Records and force sums are those of the two launches bit for bit; the
-// records are not written to memory.  The epilogue's FP64 registers allow
-// two blocks per CU, so the launch is fused only when its blocks fit one
-// round of them: beyond that the separate launches are faster.
+// records are not written to memory.  The epilogue keeps all (up to eight)
+// splits' partial loads in flight, and its FP64 registers then allow one
+// block per CU, so the launch is fused only when its blocks fit one round of
+// them: beyond that the separate launches are faster.
 constexpr int kFuseMaxPts = 256;
-constexpr int kFuseMaxBlocks = 512;
+constexpr int kFuseMaxBlocks = 256;
 
 __host__ __device__ inline int fused_lds_floats(const TableView &t)
 {
@@ -771,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void k_force_epi(ForceArgs a, EpilogueArgs 
     float4 *laux = lrec + 4 * kFuseMaxPts;
     for (int t = threadIdx.x; t < np; t += kBlock) {
         PointRecord pr;
-        epilogue_point<4>(ea, W, pbase + m0 + t, pr);
+        epilogue_point<8>(ea, W, pbase + m0 + t, pr);
         for (int k = 0; k < 4; ++k) lrec[4 * t + k] = pr.r[k];
         laux[t] = pr.aux;
     }

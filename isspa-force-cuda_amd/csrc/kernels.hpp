@@ -199,7 +199,7 @@ hipError_t launch_field(const FieldArgs &a, int nblocks, hipStream_t s);
 hipError_t launch_epilogue(const EpilogueArgs &a, hipStream_t s);
 // the epilogue and the force pass in one launch, for models whose windows all
 // fit one receiver block, whose chunks hold at most 256 points and whose force
-// grid has at most 512 blocks (table model, total force only)
+// grid has at most 256 blocks (table model, total force only)
 bool force_epi_fits(const TableView &t, int max_window_atoms, int max_chunk_points, int nblocks);
 hipError_t launch_force_epi(const ForceArgs &a, const EpilogueArgs &ea, int nblocks, hipStream_t s);
 hipError_t launch_force(const ForceArgs &a, int nblocks, bool split, hipStream_t s);
