@@ -488,7 +488,8 @@ extern "C" int isspa_model_get_field(isspa_model *m, float *pts4_d, double *g_d,
 static void nonbond_free(nonbond_model *m)
 {
     dfree(m->aux); dfree(m->lj); dfree(m->excl); dfree(m->excl_ptr); dfree(m->excl_idx);
-    dfree(m->pbc_flag); dfree(m->atom_win); dfree(m->wins); dfree(m->blk); dfree(m->part);
+    dfree(m->pbc_flag); dfree(m->atom_win); dfree(m->wins); dfree(m->blk); dfree(m->blk_tot);
+    dfree(m->part);
 }
 
 extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const int *nbparm,
@@ -601,14 +602,58 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
             for (int ch = 0; ch < d.nbc; ++ch) bl.push_back({(long long)d.apc, make_int4(w, b, ch, 0)});
         m->wins_h[w] = d;
     }
-    std::stable_sort(bl.begin(), bl.end(),
-                     [](const std::pair<long long, int4> &a, const std::pair<long long, int4> &b) {
-                         return a.first > b.first;
-                     });
-    std::vector<int4> blv;
-    for (auto &x : bl) blv.push_back(x.second);
+    // total force: a window of kSymMinAtoms or more takes each unordered pair
+    // once in super-tiles of 256 x 256 atoms (and its near pairs in both
+    // orders in one block per 256 receivers) when no pair of two different
+    // blocks can be excluded (no exclusion beyond the 128-wide windows) and
+    // the LJ pair table is symmetric; its atoms then have one partial entry
+    // per 256-atom block.  Other windows keep their partner chunks.  The
+    // weights order the blocks longest first: partners per lane, a
+    // super-tile's pair costing about 1.1 of an ordered pair.
+    bool lj_sym = true;
+    for (int a = 0; a < T; ++a)
+        for (int b = 0; b < a; ++b) {
+            const float2 u = lj[(size_t)a * T + b], v = lj[(size_t)b * T + a];
+            lj_sym = lj_sym && u.x == v.x && u.y == v.y;
+        }
+    std::vector<std::pair<long long, int4>> blt;
+    long long Bt = 0;
+    for (int w = 0; w < Ly.W; ++w) {
+        WinDesc &d = m->wins_h[w];
+        const int nw = Ly.n[w];
+        const bool sym = !fallback && lj_sym && nw >= kSymMinAtoms;
+        const int nsb = (nw + kNbBlock - 1) / kNbBlock;
+        d.nte = sym ? nsb : d.nbc;
+        d.spo = Bt;
+        Bt += (long long)d.nte * nw;
+        if (!sym) {
+            for (int b = 0; b < nsb; ++b)
+                for (int ch = 0; ch < d.nbc; ++ch)
+                    blt.push_back({(long long)d.apc, make_int4(w, b, ch, kNbKindChunk)});
+            continue;
+        }
+        m->any_sym = true;
+        for (int rb = 0; rb < nsb; ++rb) {
+            const int lo = std::max(0, (rb - 1) * kNbBlock), hi = std::min(nw, (rb + 2) * kNbBlock);
+            blt.push_back({(long long)(hi - lo), make_int4(w, rb, 0, kNbKindNear)});
+            for (int pb = rb + 2; pb < nsb; ++pb)
+                blt.push_back({(long long)(1.1 * kNbBlock), make_int4(w, rb, pb, kNbKindSym)});
+        }
+    }
+    auto by_weight = [](std::vector<std::pair<long long, int4>> &v) {
+        std::stable_sort(v.begin(), v.end(),
+                         [](const std::pair<long long, int4> &a, const std::pair<long long, int4> &b) {
+                             return a.first > b.first;
+                         });
+        std::vector<int4> out;
+        for (auto &x : v) out.push_back(x.second);
+        return out;
+    };
+    std::vector<int4> blv = by_weight(bl), bltv = by_weight(blt);
     m->nblk = (int)blv.size();
     m->B = B;
+    m->nblk_tot = (int)bltv.size();
+    m->B_tot = Bt;
 #define TRY(x)                    \
     if ((rc = (x)) != ISSPA_OK) { \
         nonbond_free(m);          \
@@ -626,7 +671,8 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
     TRY(upload(&m->atom_win, Ly.atom_win.data(), Ly.atom_win.size()));
     TRY(upload(&m->wins, m->wins_h.data(), m->wins_h.size()));
     TRY(upload(&m->blk, blv.data(), blv.size()));
-    TRY(dalloc(&m->part, (size_t)9 * B));
+    TRY(upload(&m->blk_tot, bltv.data(), bltv.size()));
+    TRY(dalloc(&m->part, (size_t)std::max(9 * B, 3 * Bt)));
 #undef TRY
     *out = m;
     return ISSPA_OK;
@@ -649,12 +695,14 @@ extern "C" int nonbond_model_destroy(nonbond_model *m)
     return ISSPA_OK;
 }
 
-static NonbondArgs nonbond_args(nonbond_model *m, const float4 *x, const int *pbc_flag)
+// split: the components' block table and partial layout; otherwise the total force's
+static NonbondArgs nonbond_args(nonbond_model *m, const float4 *x, const int *pbc_flag, bool split)
 {
     NonbondArgs a{};
-    a.wins = m->wins; a.blocks = m->blk; a.xyzq = x; a.aux = m->aux; a.lj = m->lj; a.T = m->T;
+    a.wins = m->wins; a.blocks = split ? m->blk : m->blk_tot; a.xyzq = x; a.aux = m->aux;
+    a.lj = m->lj; a.T = m->T;
     a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
-    a.pbc_flag = pbc_flag; a.part = m->part; a.B = m->B;
+    a.pbc_flag = pbc_flag; a.part = m->part; a.B = split ? m->B : m->B_tot;
     return a;
 }
 
@@ -669,15 +717,16 @@ int nonbond_partials(nonbond_model *m, const float *xyzq4_d, bool split, hipStre
         HIPCHK(launch_pbc_flag(m->wins, m->lay.W, x, 0.0f, m->pbc_flag, s));
         pbc_flag = m->pbc_flag;
     }
-    HIPCHK(launch_nonbond(nonbond_args(m, x, pbc_flag), m->nblk, split, s));
+    HIPCHK(launch_nonbond(nonbond_args(m, x, pbc_flag, split), split ? m->nblk : m->nblk_tot,
+                          split, s));
     return ISSPA_OK;
 }
 
 int nonbond_reduce(nonbond_model *m, bool split, float4 *f, double *o0, double *o1, double *o2,
                    hipStream_t s)
 {
-    HIPCHK(launch_chunk_reduce3(m->wins, m->atom_win, m->part, m->B, m->n, split ? 9 : 3, f, o0,
-                                o1, o2, s));
+    HIPCHK(launch_chunk_reduce3(m->wins, m->atom_win, m->part, split ? m->B : m->B_tot, m->n,
+                                split ? 9 : 3, f, o0, o1, o2, s));
     return ISSPA_OK;
 }
 
@@ -1181,13 +1230,15 @@ static int engine_forces(md_engine *e, hipStream_t s, bool integrate)
     }
     HIPCHK(launch_step_prologue(p, s));
     // without the side stream the all-pairs blocks ride on the field launch,
-    // reusing the prologue's IS-SPA minimum-image flags
-    const bool nb_in_field = m && e->nb && !fork;
+    // reusing the prologue's IS-SPA minimum-image flags (the field kernel
+    // carries partner-chunk blocks only: windows large enough for super-tiles
+    // always have the side stream)
+    const bool nb_in_field = m && e->nb && !fork && !e->nb->any_sym;
     if (m) {
         FieldArgs fa = field_args(m, x4);
         if (nb_in_field) {
-            fa.nb = nonbond_args(e->nb, x4, m->pbc_flag);
-            fa.nb_blocks = e->nb->nblk;
+            fa.nb = nonbond_args(e->nb, x4, m->pbc_flag, false);
+            fa.nb_blocks = e->nb->nblk_tot;
         }
         HIPCHK(launch_field(fa, m->nfblk, s));
         const ForceArgs fo = force_args(m, x4);
@@ -1208,7 +1259,7 @@ static int engine_forces(md_engine *e, hipStream_t s, bool integrate)
         a.nb_wins = e->nb->wins;
         a.nb_atom_win = e->nb->atom_win;
         a.nb_part = e->nb->part;
-        a.nb_B = e->nb->B;
+        a.nb_B = e->nb->B_tot;
     }
     if (integrate) {
         // the assembly launch also integrates each atom and advances the step

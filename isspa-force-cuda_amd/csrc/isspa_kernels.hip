@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     extern __shared__ float lds_tab[];
     if (NB && (int)blockIdx.x >= a.nfield) {
         // an all-pairs block riding on this launch (its LJ table in the LDS)
-        nonbond_block<false>(a.nb, blockIdx.x - a.nfield, reinterpret_cast<float2 *>(lds_tab));
+        nonbond_block<false, false>(a.nb, blockIdx.x - a.nfield, reinterpret_cast<float2 *>(lds_tab));
         return;
     }
     const int4 bt = a.blocks[blockIdx.x];

@@ -26,7 +26,9 @@ struct WinDesc {
     long long npo;      // nonbond partial offset: sum of nbc * n
     uint32_t sys;       // Philox counter word 3
     float L, h;         // box length and half length
-    int pad;
+    int nte;            // nonbond total force: entries per atom (nbc, or the
+                        // number of 256-atom blocks when pairs are taken once)
+    long long spo;      // nonbond total-force partial offset: sum of nte * n
 };
 
 // Per-type tables on the device, each type padded with one trailing zero row
@@ -56,9 +58,17 @@ struct TableView {
 constexpr int kRecFloats = 16;
 constexpr int kAuxFloats = 4;
 
+// all-pairs blocks: 256 receivers; kinds of the block table's .w
+constexpr int kNbBlock = 256;
+constexpr int kNbKindChunk = 0;  // receivers x a partner chunk, every pair in both orders
+constexpr int kNbKindNear = 1;   // receivers x their own and the two neighbouring blocks
+constexpr int kNbKindSym = 2;    // super-tile: receiver block x partner block, each pair once
+constexpr int kSymMinAtoms = 4096;            // windows that take each pair once
+constexpr int kSymLdsDoubles = 3 * kNbBlock;  // a super-tile's partner sums in LDS
+
 struct NonbondArgs {
     const WinDesc *wins;
-    const int4 *blocks;    // {w, receiver block, partner chunk, -}
+    const int4 *blocks;    // {w, receiver block, partner chunk or block, kind}
     const float4 *xyzq;
     const float4 *aux;     // per atom j: {4 R_j^2, cA_j, cB_j, LJ type (int bits)}
     const float2 *lj;      // [T*T] {12A, 6B}
@@ -68,7 +78,7 @@ struct NonbondArgs {
     const int *excl_idx;
     float cfar;
     const int *pbc_flag;
-    double *part;          // [comps][B]
+    double *part;          // [comps][B], in the layout of the launch's mode
     long long B;
 };
 

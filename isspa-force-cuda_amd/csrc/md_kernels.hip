@@ -22,17 +22,20 @@ template <bool SPLIT>
 __global__ __launch_bounds__(kBlk) void k_nonbond(NonbondArgs a)
 {
     extern __shared__ float2 lds_lj[];
-    nonbond_block<SPLIT>(a, blockIdx.x, lds_lj);
+    nonbond_block<SPLIT, !SPLIT>(a, blockIdx.x, lds_lj);
 }
 
 hipError_t launch_nonbond(const NonbondArgs &a, int nb, bool split, hipStream_t s)
 {
     if (nb == 0) return hipSuccess;
-    const size_t lds = sizeof(float2) * a.T * a.T;
-    if (split)
-        hipLaunchKernelGGL(k_nonbond<true>, dim3(nb), dim3(kBlk), lds, s, a);
-    else
+    if (split) {
+        hipLaunchKernelGGL(k_nonbond<true>, dim3(nb), dim3(kBlk), sizeof(float2) * a.T * a.T, s,
+                           a);
+    } else {
+        // the LJ table, then the super-tiles' partner sums
+        const size_t lds = sizeof(float2) * ((a.T * a.T + 1) & ~1) + sizeof(double) * kSymLdsDoubles;
         hipLaunchKernelGGL(k_nonbond<false>, dim3(nb), dim3(kBlk), lds, s, a);
+    }
     return hipGetLastError();
 }
 

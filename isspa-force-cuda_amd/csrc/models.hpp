@@ -76,10 +76,17 @@ struct nonbond_model {
     int *pbc_flag = nullptr, *atom_win = nullptr;
     std::vector<WinDesc> wins_h;
     WinDesc *wins = nullptr;
+    // split components: every pair in both orders, one entry per partner chunk
     int4 *blk = nullptr;
     int nblk = 0;
     long long B = 0;
-    double *part = nullptr;
+    // total force: windows of kSymMinAtoms or more take each unordered pair
+    // once (super-tiles); the others keep their partner chunks
+    int4 *blk_tot = nullptr;
+    int nblk_tot = 0;
+    long long B_tot = 0;
+    bool any_sym = false;
+    double *part = nullptr;  // max(9 B, 3 B_tot) doubles, shared by both layouts
 };
 
 // the {cp E0, A1} array stored after the point records

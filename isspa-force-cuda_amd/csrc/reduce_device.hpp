@@ -85,8 +85,9 @@ __device__ inline void bonded_slice_sum(const float4 *__restrict__ contrib,
     }
 }
 
-// all-pairs partials of atom i (COMPS = 3, or 9 in split mode), partner
-// chunks of slice sl, added in chunk order
+// all-pairs partials of atom i (COMPS = 3: the total force, entries of the
+// total-force layout; 9: the split components, one entry per partner chunk),
+// entries of slice sl, added in entry order
 template <int COMPS>
 __device__ inline void nonbond_slice_sum(const WinDesc *__restrict__ wins,
                                          const int *__restrict__ atom_win,
@@ -94,9 +95,10 @@ __device__ inline void nonbond_slice_sum(const WinDesc *__restrict__ wins,
                                          int sl, double (&s)[COMPS])
 {
     const WinDesc W = wins[atom_win[i]];
-    const long long base = W.npo + (i - W.a0);
+    const long long base = (COMPS == 3 ? W.spo : W.npo) + (i - W.a0);
+    const int ne = COMPS == 3 ? W.nte : W.nbc;
     int c = sl;
-    for (; c + (kSumBatch - 1) * kSumSlices < W.nbc; c += kSumBatch * kSumSlices) {
+    for (; c + (kSumBatch - 1) * kSumSlices < ne; c += kSumBatch * kSumSlices) {
         double v[kSumBatch][COMPS];
 #pragma unroll
         for (int u = 0; u < kSumBatch; ++u) {
@@ -109,7 +111,7 @@ __device__ inline void nonbond_slice_sum(const WinDesc *__restrict__ wins,
 #pragma unroll
             for (int k = 0; k < COMPS; ++k) s[k] += v[u][k];
     }
-    for (; c < W.nbc; c += kSumSlices) {
+    for (; c < ne; c += kSumSlices) {
         const long long o = base + (long long)c * W.n;
 #pragma unroll
         for (int k = 0; k < COMPS; ++k) s[k] += part[k * B + o];
