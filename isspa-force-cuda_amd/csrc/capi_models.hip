@@ -603,13 +603,14 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
         m->wins_h[w] = d;
     }
     // total force: a window of kSymMinAtoms or more takes each unordered pair
-    // once in super-tiles of 256 x 256 atoms (and its near pairs in both
-    // orders in one block per 256 receivers) when no pair of two different
-    // blocks can be excluded (no exclusion beyond the 128-wide windows) and
-    // the LJ pair table is symmetric; its atoms then have one partial entry
-    // per 256-atom block.  Other windows keep their partner chunks.  The
-    // weights order the blocks longest first: partners per lane, a
-    // super-tile's pair costing about 1.1 of an ordered pair.
+    // once in super-tiles of 256 x 256 atoms (and the pairs of a block with
+    // itself and its neighbours in both orders, one ordered block per
+    // receiver block and partner block) when no pair of two blocks at least
+    // two apart can be excluded (no exclusion beyond the 128-wide windows)
+    // and the LJ pair table is symmetric; its atoms then have one partial
+    // entry per 256-atom block.  Other windows keep their partner chunks.
+    // The weights order the blocks longest first: partners per lane, a
+    // super-tile's pair step costing about 1.25 of an ordered pair.
     bool lj_sym = true;
     for (int a = 0; a < T; ++a)
         for (int b = 0; b < a; ++b) {
@@ -634,10 +635,11 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
         }
         m->any_sym = true;
         for (int rb = 0; rb < nsb; ++rb) {
-            const int lo = std::max(0, (rb - 1) * kNbBlock), hi = std::min(nw, (rb + 2) * kNbBlock);
-            blt.push_back({(long long)(hi - lo), make_int4(w, rb, 0, kNbKindNear)});
+            for (int q = std::max(0, rb - 1); q <= std::min(nsb - 1, rb + 1); ++q)
+                blt.push_back({(long long)std::min(kNbBlock, nw - q * kNbBlock),
+                               make_int4(w, rb, q, kNbKindNear)});
             for (int pb = rb + 2; pb < nsb; ++pb)
-                blt.push_back({(long long)(1.1 * kNbBlock), make_int4(w, rb, pb, kNbKindSym)});
+                blt.push_back({(long long)(1.25 * kNbBlock), make_int4(w, rb, pb, kNbKindSym)});
         }
     }
     auto by_weight = [](std::vector<std::pair<long long, int4>> &v) {

@@ -61,7 +61,7 @@ constexpr int kAuxFloats = 4;
 // all-pairs blocks: 256 receivers; kinds of the block table's .w
 constexpr int kNbBlock = 256;
 constexpr int kNbKindChunk = 0;  // receivers x a partner chunk, every pair in both orders
-constexpr int kNbKindNear = 1;   // receivers x their own and the two neighbouring blocks
+constexpr int kNbKindNear = 1;   // receiver block x itself or a neighbouring block, ordered
 constexpr int kNbKindSym = 2;    // super-tile: receiver block x partner block, each pair once
 constexpr int kSymMinAtoms = 4096;            // windows that take each pair once
 constexpr int kSymLdsDoubles = 3 * kNbBlock;  // a super-tile's partner sums in LDS

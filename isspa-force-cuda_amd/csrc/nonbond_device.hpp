@@ -339,11 +339,11 @@ __device__ __forceinline__ void nonbond_block(const NonbondArgs &a, int blk, flo
     }
     __syncthreads();
     const bool near_kind = SYM && bt.w == kNbKindNear;
-    // partners: a chunk, or (near kind) the receiver block and its neighbours
-    const int j0 = near_kind ? W.a0 + max(0, (bt.y - 1) * kNbBlock) : W.a0 + bt.z * W.apc;
-    const int j1 = near_kind ? W.a0 + min(W.n, (bt.y + 2) * kNbBlock)
-                             : min(W.a0 + W.n, j0 + W.apc);
-    const int entry = near_kind ? bt.y : bt.z;
+    // partners: a chunk, or (near kind) 256-atom block bt.z, the receivers'
+    // own block or a neighbour; both write entry bt.z
+    const int j0 = W.a0 + bt.z * (near_kind ? kNbBlock : W.apc);
+    const int j1 = min(W.a0 + W.n, j0 + (near_kind ? kNbBlock : W.apc));
+    const int entry = bt.z;
     const int il = bt.y * kNbBlock + threadIdx.x;
     const bool valid = il < W.n;
     const int i = W.a0 + (valid ? il : W.n - 1);  // idle lanes mirror the last atom, results dropped
@@ -382,14 +382,6 @@ __device__ __forceinline__ void nonbond_block(const NonbondArgs &a, int blk, flo
     const long long o = (SPLIT ? W.npo : W.spo) + (long long)entry * W.n + il;
     const int comps = SPLIT ? 9 : 3;
     for (int k = 0; k < comps; ++k) a.part[k * a.B + o] = acc[k];
-    if (near_kind) {
-        // the neighbouring blocks' entries hold no super-tile: zero
-        const long long base = W.spo + il;
-        if (bt.y > 0)
-            for (int k = 0; k < 3; ++k) a.part[k * a.B + base + (long long)(bt.y - 1) * W.n] = 0.0;
-        if (bt.y + 1 < W.nte)
-            for (int k = 0; k < 3; ++k) a.part[k * a.B + base + (long long)(bt.y + 1) * W.n] = 0.0;
-    }
 }
 
 }  // namespace isspa

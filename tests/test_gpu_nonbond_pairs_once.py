@@ -2,8 +2,8 @@
 
 Windows of 4,096 atoms or more take their total all-pairs force from 256 x
 256-atom super-tiles (every pair of two blocks at least two apart, evaluated
-once and added to both atoms) plus one ordered block per 256 receivers for
-the pairs within a block and its two neighbours.  The split components keep
+once and added to both atoms) plus ordered blocks for the pairs of a block
+with itself and with its two neighbours.  The split components keep
 every pair in both orders.  These tests check the total against the FP64
 oracle (src/nonbond_force_cuda.cu:31-174 restated), in the open and in a
 periodic box, at 4,400 atoms (a ragged last block of 48) and at the 19,998-atom
