@@ -131,16 +131,19 @@ __device__ __forceinline__ void nonbond_tile(const NonbondArgs &a, int i, int j0
 // are all more than 256 indices apart, so none is excluded (the windows hold
 // every exclusion) and none is a self pair.  Each pair is evaluated once and
 // its force added to both atoms.  The pairs of a block with itself and with
-// its two neighbours are evaluated in both orders by one ordered block per
-// receiver block, which also tests the exclusions.
+// its two neighbours are evaluated in both orders by ordered blocks (one per
+// receiver block and partner block), which also test the exclusions.  Atom
+// a of block A thus has one partial entry per block q: from super-tile
+// (q, A) for q <= A - 2, from the ordered block (A, q) for |q - A| <= 1, and
+// from super-tile (A, q) for q >= A + 2.
 //
 // Inside a super-tile each 16-lane row of a wave holds 16 receivers (one a
 // lane, in VGPRs) and, in round k, the 16 partners of group (row + k) mod 16
 // (one a lane).  Step s of a round pairs lane c with the partner of lane
 // (c + s) mod 16, read by a DPP row rotation, so no pair operand is
 // wave-uniform and the pair loop reads no SGPR.  The partner-side force
-// accumulators travel with their partners: each step rotates them by one
-// lane (v_sub_f32 with a DPP operand) before adding the step's term.  After
+// accumulators travel with their partners: each step moves them one lane
+// (a DPP move) and adds the step's term to them.  After
 // 16 rounds every row has met every partner group once; the rows' partner
 // sums go to LDS slots in round order, one barrier a round.
 //
