@@ -202,6 +202,11 @@ int nonbond_model_create_batch(int n_atoms, int n_lj_types, const int *ityp, con
                                int n_isspa_types, const float *rmax, float ep, float lbox,
                                const isspa_windows *windows, nonbond_model **out);
 int nonbond_model_destroy(nonbond_model *m);
+/* f4_d float4[N] += the total all-pairs force.  Windows of 4,096 atoms or more
+ * (with a symmetric LJ pair table and every exclusion within 64 indices)
+ * evaluate each unordered pair once and add it to both atoms; the result
+ * agrees with the sum of the components below to FP32 rounding and is the
+ * same bit for bit from run to run. */
 int nonbond_force(nonbond_model *m, const float *xyzq4_d, float *f4_d, void *stream);
 /* components in FP64 (double[N*3] each, overwritten): LJ, Coulomb, IS-SPA pair */
 int nonbond_force_components(nonbond_model *m, const float *xyzq4_d, double *f_lj_d,
