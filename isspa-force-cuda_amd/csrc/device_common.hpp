@@ -100,4 +100,16 @@ __device__ __forceinline__ float rsqrt_f32(float x) { return __builtin_amdgcn_rs
 // wave-uniform "any lane" test
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
 
+// A read-only input seen through the constant address space.  Its loads at a
+// wave-uniform index are scalar loads whatever else the kernel writes: the
+// compiler otherwise turns them into per-lane vector loads when it cannot
+// prove that no store of the kernel reaches them (as in a kernel that also
+// runs the all-pairs blocks, which write their partials).  Only for memory
+// that no work-item writes during the launch.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *read_only(const T *p)
+{
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
 }  // namespace isspa

@@ -151,10 +151,12 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
                                            float px, float py, float pz, float R2, float &ax,
                                            float &ay, float &az, FieldRare &st)
 {
-    const float4 *__restrict__ xq = a.xyzq;
+    const auto *xq = read_only(reinterpret_cast<const float *>(a.xyzq));
+    const auto *ity = read_only(a.itype);
 #pragma unroll 2
     for (int j = j0; j < j1; ++j) {
-        const float4 q = xq[j];  // wave-uniform
+        // wave-uniform
+        const float4 q = make_float4(xq[4 * j], xq[4 * j + 1], xq[4 * j + 2], xq[4 * j + 3]);
         float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
         if (PBC) {
             rx = min_image1(rx, L, h);
@@ -171,9 +173,9 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
         az = __builtin_fmaf(so, rz, az);
         if (in) {
             if (AN)
-                field_in_sphere_an(dtypes[a.itype[j]], d2, inv, s, rx, ry, rz, a.e0, st);
+                field_in_sphere_an(dtypes[ity[j]], d2, inv, s, rx, ry, rz, a.e0, st);
             else
-                field_in_sphere(a.tab, Gtab, Etab, a.itype[j], d2, inv, s, rx, ry, rz, a.e0,
+                field_in_sphere(a.tab, Gtab, Etab, ity[j], d2, inv, s, rx, ry, rz, a.e0,
                                 st);
         }
     }
