@@ -100,3 +100,35 @@ def test_pairs_once_full_size():
     print(f"  total vs oracle: rel RMS {err:.3e}", flush=True)
     assert err <= TOL
     g.close()
+
+
+def test_pairs_once_window_in_a_batch():
+    """A batch of a 4,400-atom window (super-tiles) and two small windows
+    (partner chunks): each window's total against the oracle, and the large
+    window bit for bit equal to its plain run (its blocks depend on the window
+    alone)."""
+    from isspa_amd import batch, synth
+    big = _lattice(False)
+    chain = synth.make_chain()
+    chain.xyzq[:, :3] += np.float32(100.0)
+    mid = synth.make_lattice(10, 40.0, 3, 200.0, seed=7)
+    for s in (big, chain, mid):
+        s.lbox, s.n_mc, s.temperature_k, s.pnu = 200.0, 20, 298.0, 0.01
+    wins = [chain, big, mid]
+    merged = batch.merge_windows(wins)
+    print(f"case pairs-once batch: windows {[s.n_atoms for s in wins]}", flush=True)
+    g = _gpu(merged, isspa=False, bonded=False, us=False)
+    g.zero_forces()
+    g.nonbond_force()
+    tot = g.f[:, :3].clone()
+    g.close()
+    for s, (a, b) in zip(wins, batch.window_slices(merged)):
+        err = max_abs_rel_rms(tot[a:b].double().cpu().numpy(), po.nonbond(s)["total"])
+        print(f"  window of {s.n_atoms}: rel RMS {err:.3e}", flush=True)
+        assert err <= TOL
+    p = _gpu(big, isspa=False, bonded=False, us=False)
+    p.zero_forces()
+    p.nonbond_force()
+    a, b = batch.window_slices(merged)[1]
+    assert bool((p.f[:, :3] == tot[a:b]).all())
+    p.close()
