@@ -617,12 +617,14 @@ extern "C" int nonbond_model_create_batch(int n, int T, const int *ityp, const i
             const float2 u = lj[(size_t)a * T + b], v = lj[(size_t)b * T + a];
             lj_sym = lj_sym && u.x == v.x && u.y == v.y;
         }
+    // the LJ table and a super-tile's partner sums share the block's 64 KB of LDS
+    const bool sym_lds_ok = nonbond_lds_bytes(T, true) <= 65536;
     std::vector<std::pair<long long, int4>> blt;
     long long Bt = 0;
     for (int w = 0; w < Ly.W; ++w) {
         WinDesc &d = m->wins_h[w];
         const int nw = Ly.n[w];
-        const bool sym = !fallback && lj_sym && nw >= kSymMinAtoms;
+        const bool sym = !fallback && lj_sym && sym_lds_ok && nw >= kSymMinAtoms;
         const int nsb = (nw + kNbBlock - 1) / kNbBlock;
         d.nte = sym ? nsb : d.nbc;
         d.spo = Bt;
@@ -705,6 +707,7 @@ static NonbondArgs nonbond_args(nonbond_model *m, const float4 *x, const int *pb
     a.lj = m->lj; a.T = m->T;
     a.excl = m->excl; a.excl_ptr = m->excl_ptr; a.excl_idx = m->excl_idx; a.cfar = m->cfar;
     a.pbc_flag = pbc_flag; a.part = m->part; a.B = split ? m->B : m->B_tot;
+    a.sym = !split && m->any_sym;
     return a;
 }
 

@@ -80,7 +80,16 @@ struct NonbondArgs {
     const int *pbc_flag;
     double *part;          // [comps][B], in the layout of the launch's mode
     long long B;
+    int sym;               // the block table holds super-tiles
 };
+
+// dynamic LDS of an all-pairs block: the T x T LJ table, then (super-tiles)
+// the partner sums on an 8-byte boundary
+inline size_t nonbond_lds_bytes(int T, bool sym)
+{
+    return sym ? sizeof(float) * 2 * (((size_t)T * T + 1) & ~(size_t)1) + sizeof(double) * kSymLdsDoubles
+               : sizeof(float) * 2 * (size_t)T * T;
+}
 
 struct FieldArgs {
     const WinDesc *wins;

@@ -28,14 +28,11 @@ __global__ __launch_bounds__(kBlk) void k_nonbond(NonbondArgs a)
 hipError_t launch_nonbond(const NonbondArgs &a, int nb, bool split, hipStream_t s)
 {
     if (nb == 0) return hipSuccess;
-    if (split) {
-        hipLaunchKernelGGL(k_nonbond<true>, dim3(nb), dim3(kBlk), sizeof(float2) * a.T * a.T, s,
-                           a);
-    } else {
-        // the LJ table, then the super-tiles' partner sums
-        const size_t lds = sizeof(float2) * ((a.T * a.T + 1) & ~1) + sizeof(double) * kSymLdsDoubles;
+    const size_t lds = nonbond_lds_bytes(a.T, a.sym != 0);
+    if (split)
+        hipLaunchKernelGGL(k_nonbond<true>, dim3(nb), dim3(kBlk), lds, s, a);
+    else
         hipLaunchKernelGGL(k_nonbond<false>, dim3(nb), dim3(kBlk), lds, s, a);
-    }
     return hipGetLastError();
 }
 

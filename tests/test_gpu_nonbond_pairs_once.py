@@ -132,3 +132,45 @@ def test_pairs_once_window_in_a_batch():
     a, b = batch.window_slices(merged)[1]
     assert bool((p.f[:, :3] == tot[a:b]).all())
     p.close()
+
+
+def _many_lj_types(s, T=89):
+    """The same system with its LJ types spread over T types of identical
+    parameters (type t becomes t + base * k): the forces are unchanged, and
+    the T x T table with a super-tile's partner sums exceeds a block's 64 KB
+    of LDS, so the total force keeps the ordered path."""
+    s = copy.deepcopy(s)
+    base = s.n_lj_types
+    reps = (T - 1) // base
+    it = s.ityp.astype(np.int64)
+    new_t = it + base * (np.arange(s.n_atoms) % reps)
+
+    def tri(a, b):
+        lo, hi = min(a, b), max(a, b)
+        return hi * (hi + 1) // 2 + lo
+
+    nbparm = np.empty(T * T, np.int32)
+    lj_a = np.zeros(T * (T + 1) // 2, np.float32)
+    lj_b = np.zeros(T * (T + 1) // 2, np.float32)
+    for a in range(T):
+        for b in range(T):
+            k = tri(a, b)
+            nbparm[a * T + b] = k
+            src = s.nbparm[(a % base) * base + (b % base)]
+            lj_a[k], lj_b[k] = s.lj_a[src], s.lj_b[src]
+    s.ityp = new_t.astype(s.ityp.dtype)
+    s.n_lj_types, s.nbparm, s.lj_a, s.lj_b = T, nbparm, lj_a, lj_b
+    return s
+
+
+def test_many_lj_types_keep_the_ordered_path():
+    s = _many_lj_types(_lattice(False))
+    print(f"case many LJ types: N={s.n_atoms} T={s.n_lj_types}", flush=True)
+    ref = po.nonbond(s)
+    g = _gpu(s, isspa=False, bonded=False)
+    g.zero_forces()
+    g.nonbond_force()
+    err = max_abs_rel_rms(g.f[:, :3].double().cpu().numpy(), ref["total"])
+    print(f"  total vs oracle: rel RMS {err:.3e}", flush=True)
+    assert err <= TOL
+    g.close()
