@@ -185,7 +185,7 @@ static int diff_isspa_forward(isspa_diff *m, const double *q_d, double *f_d, hip
                             im->lay.W, im->rmax_all, im->pbc_flag));
     FieldArgs fa = field_args(im, m->xq);
     fa.dtypes = m->dtypes;
-    HIPCHK(launch_field(fa, im->nfblk, s));
+    HIPCHK(launch_field(fa, im->ngblk, s));
     // the adjoint reads the per-point g, count and E
     HIPCHK(launch_epilogue(epilogue_args(im, true), s));
     ForceArgs ka = force_args(im, m->xq);

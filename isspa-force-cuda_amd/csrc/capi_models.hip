@@ -98,7 +98,8 @@ static void isspa_free(isspa_model *m)
     dfree(m->itype); dfree(m->atom_win); dfree(m->rmax); dfree(m->rmax2); dfree(m->vtot);
     dfree(m->g); dfree(m->e); dfree(m->f);
     dfree(m->pts); dfree(m->rec); dfree(m->g_out); dfree(m->E_out); dfree(m->E0_out);
-    dfree(m->cnt_out); dfree(m->pbc_flag); dfree(m->wins); dfree(m->fblk); dfree(m->pblk);
+    dfree(m->cnt_out); dfree(m->pbc_flag); dfree(m->wins); dfree(m->fblk); dfree(m->gblk);
+    dfree(m->pblk);
     dfree(m->part_g); dfree(m->part_cnt); dfree(m->part_s0); dfree(m->part_ein); dfree(m->part_f);
     dfree(m->prof_f);
 }
@@ -118,9 +119,10 @@ static int isspa_tile(isspa_model *m)
                                std::max(kMinAutoWaves, (long)(m->P / kPointsPerWave)));
     m->wins_h.assign(L.W, WinDesc{});
     m->max_win_atoms = m->max_chunk_pts = 0;
+    m->max_fg = 1;
     m->min_win_atoms = 1 << 30;
     long long F = 0, C = 0;
-    std::vector<std::pair<long long, int4>> fb, pb;
+    std::vector<std::pair<long long, int4>> fb, gb, pb;
     for (int w = 0; w < L.W; ++w) {
         WinDesc d = base_desc(L, w);
         const long long n = L.n[w], P = n * nmc;
@@ -146,6 +148,13 @@ static int isspa_tile(isspa_model *m)
         const int pblocks = (int)((P + 255) / 256), rblocks = (int)((n + 255) / 256);
         for (int b = 0; b < pblocks; ++b)
             for (int sp = 0; sp < d.splits; ++sp) fb.push_back({(long long)d.aps, make_int4(w, b, sp, 0)});
+        // the field pass: fg splits of the same 256 / fg points a block
+        d.fg = field_group(d.splits);
+        m->max_fg = std::max(m->max_fg, d.fg);
+        const int gpts = 256 / d.fg, gblocks = (int)((P + gpts - 1) / gpts);
+        for (int b = 0; b < gblocks; ++b)
+            for (int gr = 0; gr < d.splits / d.fg; ++gr)
+                gb.push_back({(long long)d.aps, make_int4(w, b, gr, 0)});
         // chunk-major: the receiver blocks of one point chunk are consecutive,
         // so the round-robin dispatch over the 8 XCDs gives each XCD's L2 a
         // share of them that reuses the chunk's records
@@ -157,15 +166,19 @@ static int isspa_tile(isspa_model *m)
         return a.first > b.first;
     };
     std::stable_sort(fb.begin(), fb.end(), by_work);
+    std::stable_sort(gb.begin(), gb.end(), by_work);
     std::stable_sort(pb.begin(), pb.end(), by_work);
-    std::vector<int4> fbv, pbv;
+    std::vector<int4> fbv, gbv, pbv;
     for (auto &x : fb) fbv.push_back(x.second);
+    for (auto &x : gb) gbv.push_back(x.second);
     for (auto &x : pb) pbv.push_back(x.second);
-    dfree(m->wins); dfree(m->fblk); dfree(m->pblk);
+    dfree(m->wins); dfree(m->fblk); dfree(m->gblk); dfree(m->pblk);
     CHK(upload(&m->wins, m->wins_h.data(), m->wins_h.size()));
     CHK(upload(&m->fblk, fbv.data(), fbv.size()));
+    CHK(upload(&m->gblk, gbv.data(), gbv.size()));
     CHK(upload(&m->pblk, pbv.data(), pbv.size()));
     m->nfblk = (int)fbv.size();
+    m->ngblk = (int)gbv.size();
     m->npblk = (int)pbv.size();
     m->F = F;
     m->C = C;
@@ -321,7 +334,7 @@ extern "C" int isspa_model_get_tiling(const isspa_model *m, int *field_splits, i
 FieldArgs field_args(isspa_model *m, const float4 *x)
 {
     FieldArgs fa{};
-    fa.wins = m->wins; fa.blocks = m->fblk; fa.pts = m->pts; fa.xyzq = x; fa.itype = m->itype;
+    fa.wins = m->wins; fa.blocks = m->gblk; fa.max_fg = m->max_fg; fa.pts = m->pts; fa.xyzq = x; fa.itype = m->itype;
     fa.tab = m->tv; fa.nmc = m->d.n_mc; fa.e0 = m->d.e0; fa.pbc_flag = m->pbc_flag;
     fa.part_g = m->part_g; fa.part_cnt = m->part_cnt; fa.part_s0 = m->part_s0;
     fa.part_ein = m->part_ein; fa.F = m->F;
@@ -378,7 +391,7 @@ static int isspa_field_pass(isspa_model *m, const float *xyzq4_d, uint32_t step,
     HIPCHK(launch_mc_points(m->wins, m->atom_win, x, m->itype, m->rmax, m->rmax2, m->d.n_atoms,
                             m->d.n_mc, m->d.seed, step, step_d, m->pts, s, m->lay.W, m->rmax_all,
                             m->pbc_flag));
-    HIPCHK(launch_field(field_args(m, x), m->nfblk, s));
+    HIPCHK(launch_field(field_args(m, x), m->ngblk, s));
     HIPCHK(launch_epilogue(epilogue_args(m, inspect), s));
     return ISSPA_OK;
 }
@@ -447,7 +460,7 @@ extern "C" int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int rep
                 e = launch_mc_points(m->wins, m->atom_win, x, m->itype, m->rmax, m->rmax2,
                                      m->d.n_atoms, m->d.n_mc, m->d.seed, 0u, nullptr, m->pts, s);
                 break;
-            case 1: e = launch_field(field_args(m, x), m->nfblk, s); break;
+            case 1: e = launch_field(field_args(m, x), m->ngblk, s); break;
             case 2: e = launch_epilogue(epilogue_args(m, false), s); break;
             case 3: e = launch_force(force_args(m, x), m->npblk, false, s); break;
             default:
@@ -1245,7 +1258,7 @@ static int engine_forces(md_engine *e, hipStream_t s, bool integrate)
             fa.nb = nonbond_args(e->nb, x4, m->pbc_flag, false);
             fa.nb_blocks = e->nb->nblk_tot;
         }
-        HIPCHK(launch_field(fa, m->nfblk, s));
+        HIPCHK(launch_field(fa, m->ngblk, s));
         const ForceArgs fo = force_args(m, x4);
         if (!fo.dtypes && force_epi_fits(fo.tab, m->max_win_atoms, m->max_chunk_pts, m->npblk)) {
             // small windows: the epilogue inside the force launch

@@ -74,7 +74,7 @@ extern "C" int isspa_model_force_noalloc(isspa_model *m, const float *xyzq4_d, f
     Carver cv;
     const size_t o_pts = cv.take<float4>(P);
     const size_t o_rec = cv.take<float>(P * (kRecFloats + kAuxFloats));
-    const size_t o_pg = cv.take<float>(F);
+    const size_t o_pg = cv.take<double>(F);
     const size_t o_pc = cv.take<int>(F);
     const size_t o_ps = cv.take<double>(3 * F);
     const size_t o_pe = cv.take<double>(3 * F);
@@ -91,7 +91,7 @@ extern "C" int isspa_model_force_noalloc(isspa_model *m, const float *xyzq4_d, f
     float4 *pts = reinterpret_cast<float4 *>(base + o_pts);
     float *rec = reinterpret_cast<float *>(base + o_rec);
     float4 *aux = reinterpret_cast<float4 *>(rec + P * kRecFloats);
-    float *part_g = reinterpret_cast<float *>(base + o_pg);
+    double *part_g = reinterpret_cast<double *>(base + o_pg);
     int *part_cnt = reinterpret_cast<int *>(base + o_pc);
     double *part_s0 = reinterpret_cast<double *>(base + o_ps);
     double *part_ein = reinterpret_cast<double *>(base + o_pe);
@@ -112,7 +112,7 @@ extern "C" int isspa_model_force_noalloc(isspa_model *m, const float *xyzq4_d, f
     fa.part_cnt = part_cnt;
     fa.part_s0 = part_s0;
     fa.part_ein = part_ein;
-    HIPCHK(launch_field(fa, m->nfblk, s));
+    HIPCHK(launch_field(fa, m->ngblk, s));
     EpilogueArgs ea = epilogue_args(m, false);
     ea.pts = pts;
     ea.part_g = part_g;

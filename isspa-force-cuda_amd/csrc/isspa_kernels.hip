@@ -151,12 +151,15 @@ __device__ __forceinline__ void field_tile(const FieldArgs &a, const float *Gtab
                                            float px, float py, float pz, float R2, float &ax,
                                            float &ay, float &az, FieldRare &st)
 {
-    const auto *xq = read_only(reinterpret_cast<const float *>(a.xyzq));
+    // the tile's atoms by a pointer that advances 16 bytes an atom (an index
+    // recomputed from j costs the scalar unit a sign extension and a shift
+    // before every load)
+    const auto *xq = read_only(reinterpret_cast<const float *>(a.xyzq + j0));
     const auto *ity = read_only(a.itype);
 #pragma unroll 2
-    for (int j = j0; j < j1; ++j) {
+    for (int j = j0; j < j1; ++j, xq += 4) {
         // wave-uniform
-        const float4 q = make_float4(xq[4 * j], xq[4 * j + 1], xq[4 * j + 2], xq[4 * j + 3]);
+        const float4 q = make_float4(xq[0], xq[1], xq[2], xq[3]);
         float rx = px - q.x, ry = py - q.y, rz = pz - q.z;
         if (PBC) {
             rx = min_image1(rx, L, h);
@@ -213,6 +216,17 @@ __device__ __forceinline__ void field_span(const FieldArgs &a, const float *Gtab
     }
 }
 
+// A block takes fg splits (W.fg: 1, 2 or 4) of the same 256 / fg points, one
+// split for each 256 / fg work-items, and adds its splits' sums in split order
+// through LDS: it leaves one partial per group of fg splits, so the epilogue
+// reads fg times fewer.
+struct FieldXchg {
+    double s[6];
+    float g;
+    int cnt;
+};
+constexpr int kFieldXchgBytes = 3 * 64 * (int)sizeof(FieldXchg);  // fg = 4: 3 splits of 64 points
+
 // NB: the launch also carries the all-pairs blocks after the field blocks (a
 // separate instantiation, so the plain field loop keeps its own registers)
 template <bool AN, bool NB>
@@ -227,8 +241,14 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     const int4 bt = a.blocks[blockIdx.x];
     const WinDesc W = a.wins[bt.x];
     const int P = W.n * a.nmc;
-    const int m = bt.y * kBlock + threadIdx.x;  // point within the window
-    const int split = bt.z;
+    const int fg = W.fg, np = kBlock / fg;
+    // the block's split for this work-item (a whole number of waves each),
+    // read from the first lane so the compiler knows it is wave-uniform and
+    // the atom range stays on scalar loads
+    const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / np);
+    const int lt = (int)threadIdx.x - sub * np;
+    const int m = bt.y * np + lt;  // point within the window
+    const int split = bt.z * fg + sub;
     const int j0 = W.a0 + split * W.aps;
     const int j1 = min(W.a0 + W.n, j0 + W.aps);
     const bool valid = m < P;
@@ -236,14 +256,15 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     const float4 p = valid ? a.pts[gp] : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
     const bool pbc = a.pbc_flag[bt.x] != 0;
     // Warm-up: the 128-byte lines of the split's atoms (8 atoms a line, up to
-    // 256 lines) are requested at block start, one a work-item, so that the
-    // loop's scalar atom loads find them in L2 instead of waiting for each
-    // line in turn (the positions were last written by another XCD).  The
-    // value is only kept alive by the empty asm after the loop.
-    float w0 = 0.0f;
-    {
-        const int l = (j0 >> 3) + (int)threadIdx.x;
-        if (j1 > j0 && l <= ((j1 - 1) >> 3)) w0 = a.xyzq[8 * l].x;
+    // two lines a work-item) are requested at block start, so that the loop's
+    // scalar atom loads find them in L2 instead of waiting for each line in
+    // turn (the positions were last written by another XCD).  The values are
+    // only kept alive by the empty asm after the loop.
+    float w0 = 0.0f, w1 = 0.0f;
+    if (j1 > j0) {
+        const int l0 = (j0 >> 3) + lt, last = (j1 - 1) >> 3;
+        if (l0 <= last) w0 = a.xyzq[8 * l0].x;
+        if (l0 + np <= last) w1 = a.xyzq[8 * (l0 + np)].x;
     }
     // g and E tables (or the analytic records) for the in-sphere branch: LDS
     // when they fit.  Separate call sites give each table pointer one address
@@ -263,12 +284,36 @@ __global__ __launch_bounds__(kBlock) void k_field(FieldArgs a)
     } else {
         field_span<AN>(a, a.tab.g, a.tab.e, a.dtypes, j0, j1, W, pbc, p, fs);
     }
-    asm volatile("" ::"v"(w0));
-    const FieldRare &st = fs.st;
-    const double sx = fs.sx, sy = fs.sy, sz = fs.sz, ex = fs.ex, ey = fs.ey, ez = fs.ez;
+    asm volatile("" ::"v"(w0), "v"(w1));
+    FieldRare &st = fs.st;
+    double sx = fs.sx, sy = fs.sy, sz = fs.sz, ex = fs.ex, ey = fs.ey, ez = fs.ez;
+    // the g factors of a split multiply in FP32; splits multiply in FP64 (a
+    // product over several splits can leave the FP32 range)
+    double g = (double)st.g;
+    if (fg > 1) {
+        // the group's splits in split order, through LDS (the tables are done)
+        FieldXchg *xc = reinterpret_cast<FieldXchg *>(lds_tab);
+        __syncthreads();
+        if (sub > 0) {
+            FieldXchg &e = xc[(sub - 1) * np + lt];
+            e.s[0] = sx; e.s[1] = sy; e.s[2] = sz;
+            e.s[3] = ex; e.s[4] = ey; e.s[5] = ez;
+            e.g = st.g;
+            e.cnt = st.cnt;
+        }
+        __syncthreads();
+        if (sub > 0) return;
+        for (int k = 1; k < fg; ++k) {
+            const FieldXchg &e = xc[(k - 1) * np + lt];
+            sx += e.s[0]; sy += e.s[1]; sz += e.s[2];
+            ex += e.s[3]; ey += e.s[4]; ez += e.s[5];
+            g *= (double)e.g;
+            st.cnt += e.cnt;
+        }
+    }
     if (!valid) return;
-    const long long o = W.fpo + (long long)split * P + m;
-    a.part_g[o] = st.g;
+    const long long o = W.fpo + (long long)bt.z * P + m;
+    a.part_g[o] = g;
     a.part_cnt[o] = st.cnt;
     a.part_s0[o] = sx;
     a.part_s0[a.F + o] = sy;
@@ -342,16 +387,17 @@ __device__ __forceinline__ void epilogue_point(const EpilogueArgs &a, const WinD
     const float4 p = a.pts[gp];  // requested with the partials, used at the end
     double g = 1.0, s0[3] = {0, 0, 0}, ei[3] = {0, 0, 0};
     int cnt = 0;
-    for (int sp = 0; sp < W.splits; sp += B) {
-        float pg[B];
+    const int ngr = W.splits / W.fg;  // one partial per group of splits
+    for (int sp = 0; sp < ngr; sp += B) {
+        double pg[B];
         int pc[B];
         double ps[B][3], pe[B][3];
 #pragma unroll
         for (int u = 0; u < B; ++u) {
-            pg[u] = 1.0f;
+            pg[u] = 1.0;
             pc[u] = 0;
             for (int c = 0; c < 3; ++c) ps[u][c] = pe[u][c] = 0.0;
-            if (sp + u < W.splits) {
+            if (sp + u < ngr) {
                 const long long o = W.fpo + (long long)(sp + u) * P + m;
                 pg[u] = a.part_g[o];
                 pc[u] = a.part_cnt[o];
@@ -364,8 +410,8 @@ __device__ __forceinline__ void epilogue_point(const EpilogueArgs &a, const WinD
         }
 #pragma unroll
         for (int u = 0; u < B; ++u) {
-            if (sp + u >= W.splits) break;
-            g *= (double)pg[u];
+            if (sp + u >= ngr) break;
+            g *= pg[u];
             cnt += pc[u];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -865,12 +911,14 @@ hipError_t launch_field(const FieldArgs &a0, int nblocks, hipStream_t s)
     if (a.dtypes) {
         if (nblocks == 0) return hipSuccess;
         const int nt = diff_tab_floats(a.tab);
-        const size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
+        const size_t lds = std::max<size_t>(nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0,
+                                            a.max_fg > 1 ? kFieldXchgBytes : 0);
         hipLaunchKernelGGL((k_field<true, false>), dim3(nblocks), dim3(kBlock), lds, s, a);
     } else {
         if (nblocks + a.nb_blocks == 0) return hipSuccess;
         const int nt = field_tab_floats(a.tab);
-        size_t lds = nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0;
+        size_t lds = std::max<size_t>(nt <= kTabLdsFloats ? (size_t)nt * sizeof(float) : 0,
+                                      a.max_fg > 1 ? kFieldXchgBytes : 0);
         if (a.nb_blocks) {
             lds = std::max(lds, sizeof(float2) * a.nb.T * a.nb.T);
             hipLaunchKernelGGL((k_field<false, true>), dim3(nblocks + a.nb_blocks), dim3(kBlock),

@@ -28,8 +28,15 @@ struct WinDesc {
     float L, h;         // box length and half length
     int nte;            // nonbond total force: entries per atom (nbc, or the
                         // number of 256-atom blocks when pairs are taken once)
+    int fg;             // field pass: splits per block (1, 2 or 4; divides splits);
+                        // the block's waves take fg splits of 256 / fg points
+                        // and leave one partial per group of fg splits
+    int fgpad;
     long long spo;      // nonbond total-force partial offset: sum of nte * n
 };
+
+// splits per field block for a window of `splits` splits
+inline int field_group(int splits) { return splits % 4 == 0 ? 4 : (splits % 2 == 0 ? 2 : 1); }
 
 // Per-type tables on the device, each type padded with one trailing zero row
 // so that a lerp at the last row reads 0 (the reference reads the next type's
@@ -102,7 +109,7 @@ struct FieldArgs {
     int nmc;
     float e0;
     const int *pbc_flag;  // per window
-    float *part_g;        // [F]
+    double *part_g;       // [F] product of the in-sphere g factors
     int *part_cnt;        // [F]
     double *part_s0;      // [3][F]  sum_out q r / d^3
     double *part_ein;     // [3][F]  sum_in (-e0 q / d^3 + Etab / d) r
@@ -111,6 +118,7 @@ struct FieldArgs {
     // (tables only, nb_blocks = 0: none); nfield is set by the launcher
     NonbondArgs nb;
     int nb_blocks, nfield;
+    int max_fg;  // the largest W.fg of the launch (its LDS holds the split exchange if > 1)
 };
 
 struct EpilogueArgs {
@@ -120,7 +128,7 @@ struct EpilogueArgs {
     const float4 *pts;
     long long Ptot;
     int nmc;
-    const float *part_g;
+    const double *part_g;
     const int *part_cnt;
     const double *part_s0, *part_ein;
     long long F;

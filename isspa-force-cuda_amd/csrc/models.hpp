@@ -49,11 +49,14 @@ struct isspa_model {
     long target_waves = 0;  // 0: automatic (kPointsPerWave)
     std::vector<WinDesc> wins_h;
     WinDesc *wins = nullptr;
-    int4 *fblk = nullptr, *pblk = nullptr;
-    int nfblk = 0, npblk = 0;
+    // field blocks: gblk (fg splits a block, the field pass) and fblk (one
+    // split a block, the adjoint's point pass)
+    int4 *fblk = nullptr, *pblk = nullptr, *gblk = nullptr;
+    int nfblk = 0, npblk = 0, ngblk = 0;
+    int max_fg = 1;  // the largest splits-per-field-block of the windows
     int max_win_atoms = 0, max_chunk_pts = 0;  // over the windows, for the fused force launch
     int min_win_atoms = 0;                     // the smallest window: receiver replicas
-    float *part_g = nullptr;
+    double *part_g = nullptr;
     int *part_cnt = nullptr;
     double *part_s0 = nullptr, *part_ein = nullptr;
     double *part_f = nullptr;
