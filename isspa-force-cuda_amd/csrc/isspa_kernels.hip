@@ -536,17 +536,23 @@ __device__ __forceinline__ void force_tile(const ForceArgs &a, const float4 *__r
     // evaluated, which hides the scalar load latency when few waves share a
     // SIMD; 32-bit point indices within the window keep the loop test on the
     // SALU
-    float4 n0 = rec[4 * m0 + 0], n1 = rec[4 * m0 + 1], n2 = rec[4 * m0 + 2],
-           n3 = rec[4 * m0 + 3], na = aux[m0];
+    // the next record's pointers advance one record an iteration and stop at
+    // the last one (an index recomputed from m costs the scalar unit a clamp,
+    // a sign extension and shifts before every load)
+    const float4 *nr = rec + 4 * m0, *nx = aux + m0;
+    float4 n0 = nr[0], n1 = nr[1], n2 = nr[2], n3 = nr[3], na = nx[0];
 #pragma unroll 2
     for (int m = m0; m < m1; ++m) {
         const float4 q0 = n0, q1 = n1, q2 = n2, q3 = n3, c4 = na;
-        const int mn = min(m + 1, m1 - 1);
-        n0 = rec[4 * mn + 0];
-        n1 = rec[4 * mn + 1];
-        n2 = rec[4 * mn + 2];
-        n3 = rec[4 * mn + 3];
-        na = aux[mn];
+        if (m + 1 < m1) {
+            nr += 4;
+            nx += 1;
+        }
+        n0 = nr[0];
+        n1 = nr[1];
+        n2 = nr[2];
+        n3 = nr[3];
+        na = nx[0];
         const float Ex = q0.w, Ey = q1.x, Ez = q1.y;
         const float Vx = q1.z, Vy = q1.w, Vz = q2.x;
         const float Dx = q2.y, Dy = q2.z, Dz = q2.w;
