@@ -1,5 +1,5 @@
 // isspa_model_force_noalloc: isspa_model_force with every per-call device
-// buffer (MC points, field split partials, point records, force chunk
+// buffer (MC points, field partials, point records, force chunk
 // partials, minimum-image flags) carved from one file-scope device array
 // instead of the model's own allocations.  It allocates nothing and has no
 // counterpart in the reference; it is for callers that cannot allocate device

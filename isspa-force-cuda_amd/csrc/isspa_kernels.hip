@@ -364,8 +364,9 @@ __device__ __forceinline__ void langevin_moments(double x, double inv_x, double 
     m3 = m1 - 5.0 * m2 * inv_x;
 }
 
-// One point's epilogue: its split partials summed in split order (B
-// splits' loads in flight at a time), then the FP64 Langevin moments, the
+// One point's epilogue: its field partials (one per group of W.fg splits)
+// summed in group order (B groups' loads in flight at a time), then the FP64
+// Langevin moments, the
 // 16-float force record {p, Ehat, V, D, A2, 3 A2, A3, w} and the aux entry
 // {cp E0, A1}.  The epilogue kernel writes them to memory; the fused force
 // kernel of small systems keeps them in LDS.  Both run this same code, so
@@ -457,7 +458,7 @@ __device__ __forceinline__ void epilogue_point(const EpilogueArgs &a, const WinD
     }
 }
 
-// B splits' loads in flight at a time: 8 for launches with fewer blocks than
+// B groups' loads in flight at a time: 8 for launches with fewer blocks than
 // CUs, where each batch costs a full memory latency, and 2 for larger ones,
 // where the 8-wide batch's registers would cost occupancy in an HBM-bound
 // pass
@@ -788,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs a)
 // block) into LDS, then the force loop reads the records from there.
 // Records and force sums are those of the two launches bit for bit; the
 // records are not written to memory.  The epilogue keeps all (up to eight)
-// splits' partial loads in flight, and its FP64 registers then allow one
+// split groups' partial loads in flight, and its FP64 registers then allow one
 // block per CU, so the launch is fused only when its blocks fit one round of
 // them: beyond that the separate launches are faster.
 constexpr int kFuseMaxPts = 256;
