@@ -366,11 +366,10 @@ __device__ __forceinline__ void langevin_moments(double x, double inv_x, double 
 
 // One point's epilogue: its field partials (one per group of W.fg splits)
 // summed in group order (B groups' loads in flight at a time), then the FP64
-// Langevin moments, the
-// 16-float force record {p, Ehat, V, D, A2, 3 A2, A3, w} and the aux entry
-// {cp E0, A1}.  The epilogue kernel writes them to memory; the fused force
-// kernel of small systems keeps them in LDS.  Both run this same code, so
-// the records are the same bit for bit.
+// Langevin moments, the 16-float force record {p, Ehat, V, D, A2, 3 A2, A3,
+// w} and the aux entry {cp E0, A1}.  The epilogue kernel writes them to
+// memory; the fused force kernel of small systems keeps them in LDS.  Both
+// run this same code, so the records are the same bit for bit.
 struct PointRecord {
     float4 r[4];
     float4 aux;
