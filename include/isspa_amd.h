@@ -171,7 +171,9 @@ int isspa_model_profile(isspa_model *m, const float *xyzq4_d, int reps, float *m
  * window's.  The automatic choice aims at no more than `waves` wavefronts
  * per pass, shared among windows in proportion to their work; without a
  * set_target_waves call (or after one with waves = 0) that is one wave per 5
- * MC points, between 2048 and 131072. */
+ * MC points, between 2048 and 131072.  A split count divisible by 4 (or 2)
+ * lets each field block sum 4 (or 2) splits itself, so the epilogue reads
+ * that many times fewer partials. */
 int isspa_model_set_tiling(isspa_model *m, int field_splits, int force_chunks);
 int isspa_model_get_tiling(const isspa_model *m, int *field_splits, int *force_chunks);
 int isspa_model_set_target_waves(isspa_model *m, int waves);
