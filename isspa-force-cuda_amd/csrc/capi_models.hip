@@ -129,6 +129,9 @@ static int isspa_tile(isspa_model *m)
         const double tgt = std::max(1.0, target * ((double)n * P / work_tot));
         int s = m->user_splits > 0 ? m->user_splits : auto_pieces(P, n, kMinSplitAtoms, tgt);
         s = (int)std::max(1LL, std::min((long long)s, n));
+        // automatic splits from 4 up come in multiples of 4, so that each
+        // field block sums four splits and the epilogue reads a quarter of them
+        if (m->user_splits <= 0 && s >= 4) s -= s % 4;
         d.aps = (int)((n + s - 1) / s);
         d.splits = (int)((n + d.aps - 1) / d.aps);
         // a chunk keeps kMinChunkPoints points for each receiver replica
