@@ -851,26 +851,21 @@ __global__ __launch_bounds__(kBlock) void k_force_epi(ForceArgs a, EpilogueArgs 
 }
 
 // F_i = -q_i vtot_i S_e - vtot_i S_lj over the chunk partials, summed in the
-// fixed slice order of isspa_slice_sum (32 atoms x 8 chunk slices a block).
+// fixed slice order of isspa_slice_sum (kSumAtoms atoms x kSumSlices chunk
+// slices a block).
 template <bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_force_reduce(ReduceArgs a)
 {
     constexpr int comps = SPLIT ? 9 : 6;
-    __shared__ double red[kSumSlices - 1][comps][kSumAtoms];
+    __shared__ double red[kSumSlices][comps][kSumAtoms];
     const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
     const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < a.ntot;
     double s[comps];
     for (int k = 0; k < comps; ++k) s[k] = 0.0;
     if (valid) isspa_slice_sum<comps>(a, i, sl, s);
-    if (sl > 0)
-        for (int k = 0; k < comps; ++k) red[sl - 1][k][al] = s[k];
-    __syncthreads();
+    combine_slices<comps>(red, al, sl, s);
     if (sl != 0 || !valid) return;
-    // slices in order; LDS reads of a few slices in flight
-#pragma unroll 4
-    for (int t = 0; t < kSumSlices - 1; ++t)
-        for (int k = 0; k < comps; ++k) s[k] += red[t][k][al];
     if (!SPLIT) {
         const float3 fi = isspa_atom_force(a, i, s);
         float4 f = a.f[i];

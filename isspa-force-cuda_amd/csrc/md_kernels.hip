@@ -46,21 +46,15 @@ __global__ __launch_bounds__(kBlk) void k_chunk_reduce3(const WinDesc *__restric
                                                         long long B, int ntot, float4 *f,
                                                         double *o0, double *o1, double *o2)
 {
-    __shared__ double red[kSumSlices - 1][COMPS][kSumAtoms];
+    __shared__ double red[kSumSlices][COMPS][kSumAtoms];
     const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
     const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < ntot;
     double s[COMPS];
     for (int k = 0; k < COMPS; ++k) s[k] = 0.0;
     if (valid) nonbond_slice_sum<COMPS>(wins, atom_win, part, B, i, sl, s);
-    if (sl > 0)
-        for (int k = 0; k < COMPS; ++k) red[sl - 1][k][al] = s[k];
-    __syncthreads();
+    combine_slices<COMPS>(red, al, sl, s);
     if (sl != 0 || !valid) return;
-    // slices in order; LDS reads of a few slices in flight
-#pragma unroll 4
-    for (int t = 0; t < kSumSlices - 1; ++t)
-        for (int k = 0; k < COMPS; ++k) s[k] += red[t][k][al];
     if constexpr (COMPS == 3) {
         if (f) {
             float4 v = f[i];
@@ -291,38 +285,27 @@ hipError_t launch_bonded_terms(const BondedArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-// A block owns 32 atoms and 8 slot slices: slice t sums slots t, t + 8, ...
-// of its atom in order, then slice 0 adds slices 1..7 in order (a fixed order
-// independent of scheduling, with 8 times the waves of one lane per atom).
+// A block owns kSumAtoms atoms and kSumSlices slot slices: slice t sums slots
+// t, t + S, ... of its atom in order, then the slices are added in order (a
+// fixed order independent of scheduling, with S times the waves of one lane
+// per atom).
 __global__ __launch_bounds__(kBlk) void k_bonded_gather(const float4 *__restrict__ contrib,
                                                         const int *__restrict__ ptr,
                                                         const int *__restrict__ idx, int n,
                                                         float4 *f)
 {
-    __shared__ double red[kSumSlices - 1][3][kSumAtoms];
+    __shared__ double red[kSumSlices][3][kSumAtoms];
     const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
     const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < n;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    if (valid) bonded_slice_sum(contrib, ptr, idx, i, sl, sx, sy, sz);
-    if (sl > 0) {
-        red[sl - 1][0][al] = sx;
-        red[sl - 1][1][al] = sy;
-        red[sl - 1][2][al] = sz;
-    }
-    __syncthreads();
+    double s[3] = {0.0, 0.0, 0.0};
+    if (valid) bonded_slice_sum(contrib, ptr, idx, i, sl, s[0], s[1], s[2]);
+    combine_slices<3>(red, al, sl, s);
     if (sl != 0 || !valid) return;
-    // slices in order; LDS reads of a few slices in flight
-#pragma unroll 4
-    for (int t = 0; t < kSumSlices - 1; ++t) {
-        sx += red[t][0][al];
-        sy += red[t][1][al];
-        sz += red[t][2][al];
-    }
     float4 v = f[i];
-    v.x += (float)sx;
-    v.y += (float)sy;
-    v.z += (float)sz;
+    v.x += (float)s[0];
+    v.y += (float)s[1];
+    v.z += (float)s[2];
     f[i] = v;
 }
 
@@ -468,7 +451,7 @@ hipError_t launch_leapfrog(const WinDesc *wins, const int *atom_win, const WinDe
 __global__ __launch_bounds__(kBlk) void k_assemble_forces(AssembleArgs a)
 {
     constexpr int nc = 12;  // bonded xyz, IS-SPA 6, all-pairs xyz
-    __shared__ double red[kSumSlices - 1][nc][kSumAtoms];
+    __shared__ double red[kSumSlices][nc][kSumAtoms];
     const int al = threadIdx.x % kSumAtoms, sl = threadIdx.x / kSumAtoms;
     const int i = blockIdx.x * kSumAtoms + al;
     const bool valid = i < a.n;
@@ -494,14 +477,8 @@ __global__ __launch_bounds__(kBlk) void k_assemble_forces(AssembleArgs a)
             for (int k = 0; k < 3; ++k) s[9 + k] = t[k];
         }
     }
-    if (sl > 0)
-        for (int k = 0; k < nc; ++k) red[sl - 1][k][al] = s[k];
-    __syncthreads();
+    combine_slices<nc>(red, al, sl, s);
     if (sl == 0 && valid) {
-        // slices in order; LDS reads of a few slices in flight
-#pragma unroll 4
-    for (int t = 0; t < kSumSlices - 1; ++t)
-            for (int k = 0; k < nc; ++k) s[k] += red[t][k][al];
         float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (a.contrib) {
             f.x += (float)s[0];

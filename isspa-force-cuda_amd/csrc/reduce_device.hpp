@@ -17,6 +17,31 @@ namespace isspa {
 constexpr int kSumAtoms = 32;
 constexpr int kSumSlices = 8;
 
+// Every slice's sums into slice 0, added in slice order (slice 0's value,
+// then slices 1, 2, ...).  Component k is added up by the work-items of slice
+// k mod kSumSlices, so each chain of LDS reads is kSumSlices long rather than
+// (kSumSlices - 1) x NC in slice 0 alone; each sum has the same additions in
+// the same order, so the same bits.  Every work-item of the block calls it
+// (two barriers).
+template <int NC>
+__device__ __forceinline__ void combine_slices(double (&red)[kSumSlices][NC][kSumAtoms], int al,
+                                               int sl, double (&s)[NC])
+{
+#pragma unroll
+    for (int k = 0; k < NC; ++k) red[sl][k][al] = s[k];
+    __syncthreads();
+    for (int k = sl; k < NC; k += kSumSlices) {
+        double acc = red[0][k][al];
+#pragma unroll 4
+        for (int t = 1; t < kSumSlices; ++t) acc += red[t][k][al];
+        red[0][k][al] = acc;
+    }
+    __syncthreads();
+    if (sl == 0)
+#pragma unroll
+        for (int k = 0; k < NC; ++k) s[k] = red[0][k][al];
+}
+
 // Chunks whose partial loads one slice keeps in flight at a time; the
 // batch only groups the loads, the adds stay in chunk order.
 constexpr int kSumBatch = 6;
